@@ -1,0 +1,103 @@
+"""Writes tests/golden/ref_cpu_uniform.json: exitance images and Neumann term
+counts produced by the reference CPU package (uniform media, FP64), built from
+its own sources by oracle/Makefile.ref into oracle/_ref/rte_cpu.
+
+Run in the build container only (it needs /root/reference to build the
+binary):  make -f oracle/Makefile.ref && python tests/golden/gen_ref_fixtures.py
+The fixture holds data only: the .par inputs and the binary's outputs.
+"""
+import json
+import os
+import re
+import struct
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+BIN = os.path.join(REPO, "oracle", "_ref", "rte_cpu")
+
+# name, grid (nx,ny,nz), bounds, mua, mus, g, n, L, beam (posX,posY,diam), propThresh
+CASES = [
+    ("config1_L3_g0.8", (8, 8, 8), (-0.4, 0.4, -0.4, 0.4, 0.0, 0.8), 0.01, 1.0, 0.8, 1.0, 3, (0.0, 0.0, 0.2), 10.0),
+    ("config1_L0_g0.8", (8, 8, 8), (-0.4, 0.4, -0.4, 0.4, 0.0, 0.8), 0.01, 1.0, 0.8, 1.0, 0, (0.0, 0.0, 0.2), 10.0),
+    ("config1_L1_g0.8", (8, 8, 8), (-0.4, 0.4, -0.4, 0.4, 0.0, 0.8), 0.01, 1.0, 0.8, 1.0, 1, (0.0, 0.0, 0.2), 10.0),
+    ("config1_L5_g0.8", (8, 8, 8), (-0.4, 0.4, -0.4, 0.4, 0.0, 0.8), 0.01, 1.0, 0.8, 1.0, 5, (0.0, 0.0, 0.2), 10.0),
+    ("config1_L3_g0", (8, 8, 8), (-0.4, 0.4, -0.4, 0.4, 0.0, 0.8), 0.01, 1.0, 0.0, 1.0, 3, (0.0, 0.0, 0.2), 10.0),
+    ("config1_L1_g0", (8, 8, 8), (-0.4, 0.4, -0.4, 0.4, 0.0, 0.8), 0.01, 1.0, 0.0, 1.0, 1, (0.0, 0.0, 0.2), 10.0),
+    ("box_6x5x4_L2_offcentre_n1.37", (6, 5, 4), (-0.3, 0.3, -0.2, 0.3, 0.0, 0.48), 0.05, 2.0, 0.5, 1.37, 2,
+     (0.1, 0.05, 0.15), 10.0),
+    ("slab_7x7x3_L4_g0.9_wide_beam", (7, 7, 3), (-0.35, 0.35, -0.35, 0.35, 0.0, 0.3), 0.1, 3.0, 0.9, 1.0, 4,
+     (0.0, 0.0, 0.35), 10.0),
+]
+
+INI = """[Runtime]
+TransFile = trans.out
+ReflectFile = refl.out
+PhaseFile = phase.out
+
+[Algorithm]
+nL = {L}
+nTerms = 30
+
+[Geometry]
+nX = {nx}
+nY = {ny}
+nZ = {nz}
+xMin = {b[0]!r}
+xMax = {b[1]!r}
+yMin = {b[2]!r}
+yMax = {b[3]!r}
+zMin = {b[4]!r}
+zMax = {b[5]!r}
+subVox = 0
+subThresh = 0.0
+propThresh = {pt!r}
+selfSubVox = 1
+
+[Phantom]
+mua = {mua!r}
+mus = {mus!r}
+g = {g!r}
+n = {n!r}
+
+[Source]
+posX = {beam[0]!r}
+posY = {beam[1]!r}
+diam = {beam[2]!r}
+magnitude = 1.0
+modulation = 0.0
+"""
+
+
+def run_case(case):
+    name, (nx, ny, nz), b, mua, mus, g, n, L, beam, pt = case
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "in.ini"), "w") as f:
+            f.write(INI.format(L=L, nx=nx, ny=ny, nz=nz, b=b, pt=pt, mua=mua, mus=mus, g=g, n=n, beam=beam))
+        out = subprocess.run([BIN, "in.ini"], cwd=d, capture_output=True, text=True, check=True,
+                             env=dict(os.environ, OMP_NUM_THREADS="8")).stdout
+        terms = int(re.findall(r"Neumann series: (\d+)", out)[-1])
+        raw = open(os.path.join(d, "trans.out"), "rb").read()
+        rx, ry = struct.unpack("ii", raw[:8])
+        vals = np.frombuffer(raw[8:], dtype=np.float64)
+        real = vals[: rx * ry].reshape(ry, rx)
+    return dict(name=name, shape=[nx, ny, nz], bounds=list(b), mua=mua, mus=mus, g=g, n=n, L=L,
+                beam=dict(pos_x=beam[0], pos_y=beam[1], diam=beam[2], mag=1.0), prop_thresh=pt,
+                terms=terms, trans_image=real.tolist())
+
+
+def main():
+    res = [run_case(c) for c in CASES]
+    with open(os.path.join(HERE, "ref_cpu_uniform.json"), "w") as f:
+        json.dump(dict(source="reference CPU package (CPU package/src), built by oracle/Makefile.ref",
+                       image_layout="trans_image[j][i], j = y index, i = x index (OutputImage order)",
+                       cases=res), f, indent=1)
+    for r in res:
+        print(r["name"], r["terms"], np.sum(r["trans_image"]))
+
+
+if __name__ == "__main__":
+    main()
