@@ -1,0 +1,1031 @@
+// rte_host.cpp -- host side of librte: the C ABI of include/rte.h, device
+// memory of contexts and media, per-medium tables and the Neumann driver.
+//
+// The driver replaces the reference's Neumann()
+// (GPU package/src/neumann.cu:23-181, FisherInfo_IO/src/neumann.cu:23-120) and
+// the CPU main loop (CPU package/src/RTE_uniform.c:79-115): the field stays on
+// the device for the whole series, every term is one propagation launch, one
+// combine launch (self block, scatter, exitance increment, stop-rule sums) and
+// one per-solve bottom-face reduction; only two scalars per solve come back to
+// the host per term, where the stop rule runs in FP64.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/rte.h"
+#include "rte_internal.hpp"
+
+using namespace rte;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(...)                                                                           \
+    do {                                                                                       \
+        hipError_t e_ = (__VA_ARGS__);                                                         \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(e_ == hipErrorOutOfMemory ? RTE_E_NOMEM : RTE_E_DEVICE, "%s: %s (%s:%d)", \
+                        #__VA_ARGS__, hipGetErrorString(e_), __FILE__, __LINE__);              \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// device buffers
+// ---------------------------------------------------------------------------
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t alloc(size_t n)
+    {
+        if (n <= bytes && p) return hipSuccess;
+        release();
+        if (n == 0) n = 16;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            return e;
+        }
+        bytes = n;
+        return hipSuccess;
+    }
+    template <typename T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+// ---------------------------------------------------------------------------
+// host-side math: SH recurrence constants, exitance weights, self-block grid
+// ---------------------------------------------------------------------------
+static double nlm(int l, int m)
+{
+    double f = 1.0;  // (l+m)!/(l-m)!
+    for (int i = l - m + 1; i <= l + m; ++i) f *= (double)i;
+    return std::sqrt((2.0 * l + 1.0) / (4.0 * M_PI * f));
+}
+
+// n_a of the scaled coefficients and gamma of the monic recurrences (see ShConst)
+static double sh_scale(int l, int m)
+{
+    double c = ((m & 1) ? -1.0 : 1.0);
+    for (int i = 2 * m - 1; i > 0; i -= 2) c *= (double)i;        // c_mm = (-1)^m (2m-1)!!
+    for (int j = m + 1; j <= l; ++j) c *= (2.0 * j - 1.0) / (double)(j - m);
+    return (m > 0 ? std::sqrt(2.0) : 1.0) * nlm(l, m) * c;
+}
+
+static double sh_scale_a(int a)
+{
+    int l = 0;
+    while ((l + 1) * (l + 1) <= a) ++l;
+    const int m = a - l * l - l;
+    return sh_scale(l, m < 0 ? -m : m);
+}
+
+template <typename T>
+static ShConst<T> make_sh_const()
+{
+    ShConst<T> c;
+    std::memset(&c, 0, sizeof c);
+    for (int m = 0; m <= kMaxL; ++m)
+        for (int l = m + 2; l <= kMaxL; ++l)
+            c.gamma[l][m] = T((double)(l + m - 1) * (double)(l - m - 1) / ((2.0 * l - 1.0) * (2.0 * l - 3.0)));
+    return c;
+}
+
+// real SH R_a(x,y,z) for a unit vector, with the recurrences of the kernels
+static void real_sh(int L, double x, double y, double z, double *out)
+{
+    static const ShConst<double> sh = make_sh_const<double>();
+    double C[kMaxL + 1], S[kMaxL + 1], P[kMaxL + 1][kMaxL + 1];
+    C[0] = 1.0;
+    S[0] = 0.0;
+    for (int m = 1; m <= L; ++m) {
+        C[m] = C[m - 1] * x - S[m - 1] * y;
+        S[m] = S[m - 1] * x + C[m - 1] * y;
+    }
+    for (int m = 0; m <= L; ++m) {
+        P[m][m] = 1.0;
+        if (m + 1 <= L) P[m + 1][m] = z;
+        for (int l = m + 2; l <= L; ++l) P[l][m] = z * P[l - 1][m] - sh.gamma[l][m] * P[l - 2][m];
+    }
+    for (int l = 0; l <= L; ++l) {
+        out[l * l + l] = sh_scale(l, 0) * P[l][0];
+        for (int m = 1; m <= l; ++m) {
+            out[l * l + l + m] = sh_scale(l, m) * P[l][m] * C[m];
+            out[l * l + l - m] = sh_scale(l, m) * P[l][m] * S[m];
+        }
+    }
+}
+
+static double legendre(int l, double x)
+{
+    if (l == 0) return 1.0;
+    double p0 = 1.0, p1 = x;
+    for (int k = 2; k <= l; ++k) {
+        double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+    }
+    return p1;
+}
+
+// Q_l = int_0^1 P_l(u) u du: the reference's 100-point left Riemann sum
+// (CPU package/src/Dist.c:190-193,211-218) or exact (Gauss-Legendre, 32 nodes)
+static void quad_weights(int L, int quad, double *Q)
+{
+    if (quad == RTE_QUAD_REF) {
+        for (int l = 0; l <= L; ++l) {
+            double s = 0.0;
+            for (int p = 0; p < 100; ++p) {
+                const double u = (double)p / 100.0;
+                s += legendre(l, u) * u / 100.0;
+            }
+            Q[l] = s;
+        }
+        return;
+    }
+    const int NG = 32;
+    double xg[NG], wg[NG];
+    for (int i = 0; i < NG; ++i) {
+        double x = std::cos(M_PI * (i + 0.75) / (NG + 0.5)), dp = 1.0;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = x;
+            for (int k = 2; k <= NG; ++k) {
+                double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = NG * (x * p1 - p0) / (x * x - 1.0);
+            const double step = p1 / dp;
+            x -= step;
+            if (std::fabs(step) < 1e-16) break;
+        }
+        xg[i] = x;
+        wg[i] = 2.0 / ((1.0 - x * x) * dp * dp);
+    }
+    for (int l = 0; l <= L; ++l) {
+        double s = 0.0;
+        for (int i = 0; i < NG; ++i) {
+            const double u = 0.5 * (xg[i] + 1.0);
+            s += 0.5 * wg[i] * legendre(l, u) * u;
+        }
+        Q[l] = s;
+    }
+}
+
+// Centre-to-face distance beta(omega) of the self-voxel quadrature on the
+// 200x200 (theta, phi) grid, following CPU package/src/Kernels_selfsub.c:815-948
+// with one sub-voxel at the centre (selfSubVox = 1, CPU package/src/RTE_uniform.c:63):
+// direction cosines and face intersections are single precision there, the
+// faces are tried x, y, z with the last accepted hit kept (also across
+// directions), and the same is done here so that beta agrees to the bit.
+static void self_beta(double dx, double dy, double dz, std::vector<double> &beta,
+                      std::vector<double> &theta, std::vector<double> &phi)
+{
+    beta.resize(kNOmega);
+    theta.resize(kNOmega);
+    phi.resize(kNOmega);
+    float hx = 0.f, hy = 0.f, hz = 0.f;  // last accepted intersection
+    int w = 0;
+    for (int t = 0; t < kAngRes; ++t) {
+        for (int p = 0; p < kAngRes; ++p, ++w) {
+            const double th = t * M_PI / kAngRes;
+            const double ph = p * 2.0 * M_PI / kAngRes;
+            theta[w] = th;
+            phi[w] = ph;
+            const float u[3] = {(float)(std::cos(ph) * std::sin(th)),
+                                (float)(std::sin(ph) * std::sin(th)), (float)std::cos(th)};
+            for (int ax = 0; ax < 3; ++ax) {
+                if (u[ax] == 0.0f) continue;
+                for (int f = 0; f < 2; ++f) {
+                    float h[3];
+                    h[ax] = f == 0 ? 1.0f : -1.0f;
+                    // the two other coordinates, in the reference's order
+                    const int o1 = ax == 0 ? 1 : (ax == 1 ? 2 : 0);
+                    const int o2 = ax == 0 ? 2 : (ax == 1 ? 0 : 1);
+                    h[o1] = (float)((double)h[ax] * (double)u[o1] / (double)u[ax]);
+                    h[o2] = (float)((double)h[ax] * (double)u[o2] / (double)u[ax]);
+                    bool inside = true;
+                    for (int c = 0; c < 3; ++c) inside = inside && h[c] <= 1.0f && h[c] >= -1.0f;
+                    if (inside && u[ax] * h[ax] >= 0.0f) {
+                        hx = h[0];
+                        hy = h[1];
+                        hz = h[2];
+                    }
+                }
+            }
+            const double ex = hx, ey = hy, ez = hz;
+            beta[w] = std::sqrt(ex * ex * (dx / 2.0) * (dx / 2.0) + ey * ey * (dy / 2.0) * (dy / 2.0) +
+                                ez * ez * (dz / 2.0) * (dz / 2.0));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// context and media
+// ---------------------------------------------------------------------------
+struct MomentSet {
+    DevBuf md, mf;  // [kNMom][K][K] in double and float
+};
+
+struct rte_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::map<std::tuple<int, double, double, double>, std::unique_ptr<MomentSet>> moments;
+    // solve workspace (grow-only)
+    DevBuf ws_W, ws_part, ws_pix, ws_sv, ws_beams, ws_red;
+    std::vector<char> host_red;
+};
+
+struct rte_medium {
+    rte_ctx *ctx = nullptr;
+    rte_grid grid{};
+    double g = 0, n = 1, c = 1;
+    int N = 0;
+    std::vector<double> mua_h, mus_h;
+    DevBuf mua_d, mus_d;          // FP64 maps
+    DevBuf mut_f, mus_f;          // FP32 mu_t, mu_s
+    DevBuf mut_d;                 // FP64 mu_t
+    DevBuf ktab_f, ktab_d;        // cached e^{-tau}/r^2 tables
+    double ktab_f_thr = -1, ktab_d_thr = -1;
+};
+
+static int set_device(rte_ctx *ctx)
+{
+    HIP_TRY(hipSetDevice(ctx->device));
+    return RTE_OK;
+}
+
+extern "C" int rte_version(void) { return 1; }
+
+extern "C" const char *rte_last_error(void) { return g_err.c_str(); }
+
+extern "C" int rte_ctx_create(int device, rte_ctx **out)
+{
+    if (!out) return fail(RTE_E_ARG, "rte_ctx_create: out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(RTE_E_ARG, "rte_ctx_create: device %d of %d", device, ndev);
+    auto ctx = std::make_unique<rte_ctx>();
+    ctx->device = device;
+    HIP_TRY(hipSetDevice(device));
+    HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&ctx->ev0));
+    HIP_TRY(hipEventCreate(&ctx->ev1));
+    *out = ctx.release();
+    return RTE_OK;
+}
+
+extern "C" int rte_ctx_destroy(rte_ctx *ctx)
+{
+    if (!ctx) return RTE_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->moments.clear();
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return RTE_OK;
+}
+
+static int check_grid(const rte_grid *g)
+{
+    if (!g) return fail(RTE_E_ARG, "grid is NULL");
+    if (g->nx < 1 || g->ny < 1 || g->nz < 1) return fail(RTE_E_ARG, "grid dims must be >= 1");
+    if ((long long)g->nx * g->ny * g->nz > (1LL << 24)) return fail(RTE_E_ARG, "grid too large");
+    if (!(g->xmax > g->xmin) || !(g->ymax > g->ymin) || !(g->zmax > g->zmin))
+        return fail(RTE_E_ARG, "grid extents must be increasing");
+    return RTE_OK;
+}
+
+static int check_optics(const rte_grid *g, const rte_optics *o)
+{
+    if (!o || !o->mu_a || !o->mu_s) return fail(RTE_E_ARG, "optics maps are NULL");
+    if (!(std::fabs(o->g) <= 1.0)) return fail(RTE_E_ARG, "|g| must be <= 1");
+    if (!(o->n > 0.0) || !(o->c > 0.0)) return fail(RTE_E_ARG, "n and c must be > 0");
+    const long long N = (long long)g->nx * g->ny * g->nz;
+    for (long long v = 0; v < N; ++v) {
+        const double a = o->mu_a[v], s = o->mu_s[v];
+        if (!(a >= 0.0) || !(s >= 0.0) || !std::isfinite(a) || !std::isfinite(s) || !(a + s > 0.0))
+            return fail(RTE_E_ARG, "invalid optics at voxel %lld (mu_a=%g, mu_s=%g)", v, a, s);
+    }
+    return RTE_OK;
+}
+
+template <typename T>
+static GridDev<T> make_grid(const rte_grid &g)
+{
+    GridDev<T> d;
+    d.nx = g.nx;
+    d.ny = g.ny;
+    d.nz = g.nz;
+    d.N = g.nx * g.ny * g.nz;
+    d.bx = (g.nx + kBrick - 1) / kBrick;
+    d.by = (g.ny + kBrick - 1) / kBrick;
+    d.bz = (g.nz + kBrick - 1) / kBrick;
+    d.nblk = d.bx * d.by * d.bz;
+    d.npad = d.nblk * kWave;
+    const double dx = (g.xmax - g.xmin) / g.nx, dy = (g.ymax - g.ymin) / g.ny,
+                 dz = (g.zmax - g.zmin) / g.nz;
+    d.dx = T(dx);
+    d.dy = T(dy);
+    d.dz = T(dz);
+    d.dV = T(dx * dy * dz);
+    return d;
+}
+
+extern "C" int rte_medium_create(rte_ctx *ctx, const rte_grid *grid, const rte_optics *optics,
+                                 rte_medium **out)
+{
+    if (!ctx || !out) return fail(RTE_E_ARG, "rte_medium_create: NULL argument");
+    *out = nullptr;
+    int rc;
+    if ((rc = check_grid(grid)) || (rc = check_optics(grid, optics))) return rc;
+    if ((rc = set_device(ctx))) return rc;
+    auto m = std::make_unique<rte_medium>();
+    m->ctx = ctx;
+    m->grid = *grid;
+    m->g = optics->g;
+    m->n = optics->n;
+    m->c = optics->c;
+    m->N = grid->nx * grid->ny * grid->nz;
+    const size_t N = (size_t)m->N;
+    m->mua_h.assign(optics->mu_a, optics->mu_a + N);
+    m->mus_h.assign(optics->mu_s, optics->mu_s + N);
+    std::vector<float> mt_f(N), ms_f(N);
+    std::vector<double> mt_d(N);
+    for (size_t v = 0; v < N; ++v) {
+        mt_d[v] = m->mua_h[v] + m->mus_h[v];
+        mt_f[v] = (float)mt_d[v];
+        ms_f[v] = (float)m->mus_h[v];
+    }
+    HIP_TRY(m->mua_d.alloc(N * 8));
+    HIP_TRY(m->mus_d.alloc(N * 8));
+    HIP_TRY(m->mut_d.alloc(N * 8));
+    HIP_TRY(m->mut_f.alloc(N * 4));
+    HIP_TRY(m->mus_f.alloc(N * 4));
+    HIP_TRY(hipMemcpy(m->mua_d.p, m->mua_h.data(), N * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(m->mus_d.p, m->mus_h.data(), N * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(m->mut_d.p, mt_d.data(), N * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(m->mut_f.p, mt_f.data(), N * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(m->mus_f.p, ms_f.data(), N * 4, hipMemcpyHostToDevice));
+    *out = m.release();
+    return RTE_OK;
+}
+
+extern "C" int rte_medium_destroy(rte_medium *m)
+{
+    if (!m) return RTE_OK;
+    (void)hipSetDevice(m->ctx->device);
+    (void)hipStreamSynchronize(m->ctx->stream);
+    delete m;
+    return RTE_OK;
+}
+
+extern "C" int rte_medium_reset_cache(rte_medium *m)
+{
+    if (!m) return fail(RTE_E_ARG, "rte_medium_reset_cache: NULL");
+    (void)hipSetDevice(m->ctx->device);
+    (void)hipStreamSynchronize(m->ctx->stream);
+    m->ktab_f.release();
+    m->ktab_d.release();
+    m->ktab_f_thr = m->ktab_d_thr = -1;
+    return RTE_OK;
+}
+
+// ---- per-(L, voxel shape) self-block moments --------------------------------
+static int get_moments(rte_ctx *ctx, int L, double dx, double dy, double dz, MomentSet **out)
+{
+    auto key = std::make_tuple(L, dx, dy, dz);
+    auto it = ctx->moments.find(key);
+    if (it != ctx->moments.end()) {
+        *out = it->second.get();
+        return RTE_OK;
+    }
+    const int K = (L + 1) * (L + 1);
+    std::vector<double> beta, th, ph;
+    self_beta(dx, dy, dz, beta, th, ph);
+    std::vector<double> R((size_t)kNOmega * K), sn(kNOmega);
+    for (int w = 0; w < kNOmega; ++w) {
+        const double st = std::sin(th[w]);
+        real_sh(L, std::cos(ph[w]) * st, std::sin(ph[w]) * st, std::cos(th[w]), &R[(size_t)w * K]);
+        sn[w] = st;
+    }
+    DevBuf dR, dsn, dbeta;
+    auto ms = std::make_unique<MomentSet>();
+    HIP_TRY(dR.alloc(R.size() * 8));
+    HIP_TRY(dsn.alloc(sn.size() * 8));
+    HIP_TRY(dbeta.alloc(beta.size() * 8));
+    HIP_TRY(ms->md.alloc((size_t)kNMom * K * K * 8));
+    HIP_TRY(ms->mf.alloc((size_t)kNMom * K * K * 4));
+    HIP_TRY(hipMemcpyAsync(dR.p, R.data(), R.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(dsn.p, sn.data(), sn.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(dbeta.p, beta.data(), beta.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(launch_moments(K, dR.as<double>(), dsn.as<double>(), dbeta.as<double>(), ms->md.as<double>(),
+                           ctx->stream));
+    std::vector<double> Mh((size_t)kNMom * K * K);
+    HIP_TRY(hipMemcpyAsync(Mh.data(), ms->md.p, Mh.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    std::vector<float> Mf(Mh.size());
+    for (size_t i = 0; i < Mh.size(); ++i) Mf[i] = (float)Mh[i];
+    HIP_TRY(hipMemcpy(ms->mf.p, Mf.data(), Mf.size() * 4, hipMemcpyHostToDevice));
+    *out = ms.get();
+    ctx->moments.emplace(key, std::move(ms));
+    return RTE_OK;
+}
+
+// ---- per-medium path tables ------------------------------------------------
+static int pick_chunks(int nblk, int nsolve, int N)
+{
+    const long long target = 4096;
+    long long c = (target + (long long)nblk * nsolve - 1) / ((long long)nblk * nsolve);
+    long long cmax = N / 32 > 0 ? N / 32 : 1;
+    if (c > cmax) c = cmax;
+    if (c < 1) c = 1;
+    return (int)c;
+}
+
+template <typename T>
+static int ensure_ktab(rte_medium *m, double thr)
+{
+    DevBuf &tab = sizeof(T) == 4 ? m->ktab_f : m->ktab_d;
+    double &tthr = sizeof(T) == 4 ? m->ktab_f_thr : m->ktab_d_thr;
+    if (tab.p && tthr == thr) return RTE_OK;
+    const GridDev<T> g = make_grid<T>(m->grid);
+    HIP_TRY(tab.alloc((size_t)g.N * g.npad * sizeof(T)));
+    const int nchunk = pick_chunks(g.nblk, 1, g.N);
+    const int clen = (g.N + nchunk - 1) / nchunk;
+    const T *mut = sizeof(T) == 4 ? (const T *)m->mut_f.p : (const T *)m->mut_d.p;
+    const T thr2 = thr > 0 ? T(thr * thr) : T(sizeof(T) == 4 ? 1e30 : 1e300);
+    HIP_TRY(launch_ktab<T>(g, (g.N + clen - 1) / clen, clen, thr2, mut, tab.as<T>(), m->ctx->stream));
+    tthr = thr;
+    return RTE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// the Neumann driver over a batch of solves of one shape
+// ---------------------------------------------------------------------------
+struct HostSolve {
+    rte_medium *m;
+    int state;  // 0 term, 2 final, 3 done
+    int n;
+    double T, sumOut, fi, lam, rel, ball;
+};
+
+static Scales make_scales(int L)
+{
+    Scales sc;
+    for (int a = 0; a < kMaxK; ++a) sc.n[a] = 1.0;
+    for (int a = 0; a < (L + 1) * (L + 1); ++a) sc.n[a] = sh_scale_a(a);
+    return sc;
+}
+
+static Scales unit_scales()
+{
+    Scales sc;
+    for (int a = 0; a < kMaxK; ++a) sc.n[a] = 1.0;
+    return sc;
+}
+
+static double stop_weight(int a)
+{
+    int l = 0;
+    while ((l + 1) * (l + 1) <= a) ++l;
+    const int m = a - l * l - l;
+    if (m == 0) return 1.0;
+    if (m > 0 && (m % 2) == 0) return std::sqrt(2.0);
+    return 0.0;
+}
+
+template <typename T, typename TA>
+static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_beam *beams,
+                     const rte_params *p, double *exitance, rte_stats *stats)
+{
+    const rte_grid &grid = media[0]->grid;
+    const int L = p->L, K = (L + 1) * (L + 1);
+    const GridDev<T> g = make_grid<T>(grid);
+    const int N = g.N, P = grid.nx * grid.ny;
+    const double dx = (grid.xmax - grid.xmin) / grid.nx, dy = (grid.ymax - grid.ymin) / grid.ny,
+                 dz = (grid.zmax - grid.zmin) / grid.nz;
+    const int path = p->path_mode;
+    const double thr = p->prop_thresh > 0 ? p->prop_thresh : -1.0;
+    const int max_terms = p->max_terms > 0 ? p->max_terms : 1000;
+    hipStream_t st = ctx->stream;
+
+    MomentSet *mom = nullptr;
+    int rc;
+    if ((rc = get_moments(ctx, L, dx, dy, dz, &mom))) return rc;
+
+    HIP_TRY(hipEventRecord(ctx->ev0, st));
+    if (path == RTE_PATH_CACHED)
+        for (int i = 0; i < nb; ++i)
+            if ((rc = ensure_ktab<T>(media[i], thr))) return rc;
+
+    const int nchunk = pick_chunks(g.nblk, nb, N);
+    const int clen = (N + nchunk - 1) / nchunk;
+    const int nch = (N + clen - 1) / clen;
+    const size_t szW = (size_t)N * K, szPart = (size_t)nch * N * K;
+    HIP_TRY(ctx->ws_W.alloc(szW * sizeof(T) * nb));
+    HIP_TRY(ctx->ws_part.alloc(szPart * sizeof(T) * nb));
+    HIP_TRY(ctx->ws_pix.alloc((size_t)3 * P * sizeof(TA) * nb));
+    HIP_TRY(ctx->ws_red.alloc((size_t)2 * sizeof(TA) * nb));
+    HIP_TRY(ctx->ws_sv.alloc(sizeof(SolveDev<T, TA>) * nb));
+    HIP_TRY(ctx->ws_beams.alloc(sizeof(BeamDev) * nb));
+
+    std::vector<SolveDev<T, TA>> sv(nb);
+    std::vector<BeamDev> bd(nb);
+    std::vector<HostSolve> hs(nb);
+    for (int i = 0; i < nb; ++i) {
+        rte_medium *m = media[i];
+        SolveDev<T, TA> &s = sv[i];
+        s.W = ctx->ws_W.as<T>() + szW * i;
+        s.part = ctx->ws_part.as<T>() + szPart * i;
+        s.ktab = sizeof(T) == 4 ? (const T *)m->ktab_f.p : (const T *)m->ktab_d.p;
+        s.mut = sizeof(T) == 4 ? (const T *)m->mut_f.p : (const T *)m->mut_d.p;
+        s.mus = sizeof(T) == 4 ? (const T *)m->mus_f.p : (const T *)m->mus_d.p;
+        s.g = T(m->g);
+        TA *pix = ctx->ws_pix.as<TA>() + (size_t)3 * P * i;
+        s.dE = pix;
+        s.bsum = pix + P;
+        s.E = pix + 2 * P;
+        s.red = ctx->ws_red.as<TA>() + 2 * i;
+        s.mode = 1;
+        // beam columns and power per column (CPU package/src/Dist.c:1090-1114, Inits.c:56)
+        const double half = (beams[i].diam + 1.0e-16) / 2.0;
+        int cols = 0;
+        for (int j = 0; j < grid.ny; ++j)
+            for (int ii = 0; ii < grid.nx; ++ii) {
+                const double xc = (double)ii * dx + grid.xmin + dx / 2.0;
+                const double yc = (double)j * dy + grid.ymin + dy / 2.0;
+                const double rad = std::sqrt((xc - beams[i].pos_x) * (xc - beams[i].pos_x) +
+                                             (yc - beams[i].pos_y) * (yc - beams[i].pos_y));
+                if (rad <= half) ++cols;
+            }
+        if (cols == 0) return fail(RTE_E_ARG, "solve %d: beam hits no voxel column", i);
+        bd[i] = BeamDev{beams[i].pos_x, beams[i].pos_y, half, beams[i].mag / (double)cols,
+                        m->mua_d.as<double>(), m->mus_d.as<double>()};
+        hs[i] = HostSolve{m, 0, 0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0};
+    }
+    SolveDev<T, TA> *svd = ctx->ws_sv.as<SolveDev<T, TA>>();
+    HIP_TRY(hipMemcpyAsync(svd, sv.data(), sizeof(SolveDev<T, TA>) * nb, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(ctx->ws_beams.p, bd.data(), sizeof(BeamDev) * nb, hipMemcpyHostToDevice, st));
+    HIP_TRY(launch_fill<T>(ctx->ws_W.as<T>(), szW * nb, T(0), st));
+    const Scales sc = make_scales(L);
+    HIP_TRY(launch_source<T, TA>(L, g, dx, dy, dz, grid.xmin, grid.ymin, sc, ctx->ws_beams.as<BeamDev>(),
+                                 svd, nb, st));
+    HIP_TRY(launch_bottom<T, TA>(P, svd, nb, st));
+    ctx->host_red.resize(sizeof(TA) * 2 * nb);
+    TA *red = reinterpret_cast<TA *>(ctx->host_red.data());
+    HIP_TRY(hipMemcpyAsync(red, ctx->ws_red.p, sizeof(TA) * 2 * nb, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (int i = 0; i < nb; ++i) {
+        HostSolve &h = hs[i];
+        h.ball = (double)red[2 * i];
+        h.T = h.ball;
+        if (p->stop_rule == RTE_STOP_LAMBDA) {
+            h.n = 1;
+            h.sumOut = (double)red[2 * i + 1];
+            h.state = (max_terms <= 1) ? 2 : 0;
+        } else {
+            h.n = 0;
+            h.state = 0;
+        }
+    }
+
+    PropArgs<T> pa;
+    pa.g = g;
+    pa.sh = make_sh_const<T>();
+    pa.nchunk = nch;
+    pa.chunk_len = clen;
+    pa.thr2 = thr > 0 ? T(thr * thr) : T(sizeof(T) == 4 ? 1e30 : 1e300);
+    CombArgs<T, TA> ca;
+    std::memset(&ca, 0, sizeof ca);
+    ca.g = g;
+    ca.nchunk = nch;
+    ca.op = 0;
+    ca.mom = sizeof(TA) == 4 ? (const TA *)mom->mf.p : (const TA *)mom->md.p;
+    ca.self_scale = TA((M_PI / kAngRes) * (2.0 * M_PI / kAngRes) / (dx * dy * dz));
+    ca.dV = TA(dx * dy * dz);
+    {
+        double Q[kMaxL + 1];
+        quad_weights(L, p->exit_quad, Q);
+        for (int l = 0; l <= L; ++l)
+            ca.coefE[l] = TA(2.0 * M_PI * dx * dy / std::sqrt(4.0 * M_PI) * std::sqrt(2.0 * l + 1.0) * Q[l]);
+        for (int a = 0; a < K; ++a) ca.wsum[a] = TA(stop_weight(a));
+        for (int a = 0; a < kMaxK; ++a) ca.nsc[a] = TA(sc.n[a]);
+    }
+
+    std::vector<int> act;
+    std::vector<SolveDev<T, TA>> svact;
+    for (;;) {
+        act.clear();
+        for (int i = 0; i < nb; ++i)
+            if (hs[i].state != 3) act.push_back(i);
+        if (act.empty()) break;
+        svact.resize(act.size());
+        for (size_t j = 0; j < act.size(); ++j) {
+            svact[j] = sv[act[j]];
+            svact[j].mode = hs[act[j]].state;
+        }
+        const int na = (int)act.size();
+        HIP_TRY(hipMemcpyAsync(svd, svact.data(), sizeof(SolveDev<T, TA>) * na, hipMemcpyHostToDevice, st));
+        HIP_TRY(launch_prop<T, TA>(L, path, pa, svd, na, st));
+        HIP_TRY(launch_combine<T, TA>(L, ca, svd, na, st));
+        HIP_TRY(launch_bottom<T, TA>(P, svd, na, st));
+        std::vector<TA> r(2 * nb);
+        HIP_TRY(hipMemcpyAsync(r.data(), ctx->ws_red.p, sizeof(TA) * 2 * nb, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        for (int j = 0; j < na; ++j) {
+            HostSolve &h = hs[act[j]];
+            const double dT = (double)r[2 * act[j]], sumW = (double)r[2 * act[j] + 1];
+            h.T += dT;
+            if (h.state == 2) {
+                h.state = 3;
+                continue;
+            }
+            if (p->stop_rule == RTE_STOP_DELTA) {
+                h.n += 1;
+                h.rel = h.T != 0.0 ? std::fabs(dT) / std::fabs(h.T) : 0.0;
+                if (h.rel < p->tol || h.n >= max_terms) h.state = 3;
+            } else {
+                // CPU package/src/Kernels_selfsub.c:1208-1229 at n + 1
+                h.sumOut += sumW;
+                h.n += 1;
+                h.rel = h.T != 0.0 ? std::fabs(dT) / std::fabs(h.T) : 0.0;
+                bool stop;
+                if (h.sumOut == 0.0) {
+                    stop = true;
+                    h.lam = 0.0;
+                } else {
+                    const double gam = sumW / h.sumOut;
+                    double hi = 1.0 - gam * (1.0 / (1.0 - gam / h.fi));
+                    h.fi = gam;
+                    if (hi > 1.0 || hi < 0.0) hi = 0.0;
+                    h.lam = hi;
+                    stop = hi > 0.9999;
+                }
+                if (stop || h.n >= max_terms) h.state = 2;
+            }
+        }
+        // the rows of svd hold the active solves; stale entries are rebuilt next pass
+    }
+    HIP_TRY(hipEventRecord(ctx->ev1, st));
+    // gather exitance images
+    std::vector<TA> img((size_t)P);
+    for (int i = 0; i < nb; ++i) {
+        HIP_TRY(hipMemcpyAsync(img.data(), sv[i].E, sizeof(TA) * P, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        for (int q = 0; q < P; ++q) exitance[(size_t)i * P + q] = (double)img[q];
+    }
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (stats) {
+        for (int i = 0; i < nb; ++i) {
+            rte_stats &s = stats[i];
+            s.terms = hs[i].n;
+            s.lambda_last = hs[i].lam;
+            s.rel_change_last = hs[i].rel;
+            s.total_exitance = hs[i].T;
+            s.ballistic_total = hs[i].ball;
+            s.gpu_ms = ms;
+        }
+    }
+    return RTE_OK;
+}
+
+static int check_params(const rte_params *p)
+{
+    if (!p) return fail(RTE_E_ARG, "params is NULL");
+    if (p->L < 0 || p->L > RTE_MAX_L) return fail(RTE_E_UNSUPP, "L=%d outside 0..%d", p->L, RTE_MAX_L);
+    if (p->stop_rule != RTE_STOP_LAMBDA && p->stop_rule != RTE_STOP_DELTA)
+        return fail(RTE_E_ARG, "unknown stop_rule %d", p->stop_rule);
+    if (p->stop_rule == RTE_STOP_DELTA && !(p->tol > 0.0)) return fail(RTE_E_ARG, "tol must be > 0");
+    if (p->precision < RTE_PREC_FP64 || p->precision > RTE_PREC_FP32)
+        return fail(RTE_E_ARG, "unknown precision %d", p->precision);
+    if (p->exit_quad != RTE_QUAD_REF && p->exit_quad != RTE_QUAD_EXACT)
+        return fail(RTE_E_ARG, "unknown exit_quad %d", p->exit_quad);
+    if (p->path_mode != RTE_PATH_CACHED && p->path_mode != RTE_PATH_MARCH)
+        return fail(RTE_E_ARG, "unknown path_mode %d", p->path_mode);
+    return RTE_OK;
+}
+
+extern "C" int rte_solve_media(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_beam *beams,
+                               const rte_params *params, double *exitance, rte_stats *stats)
+{
+    if (!ctx || !media || !beams || !exitance) return fail(RTE_E_ARG, "rte_solve_media: NULL argument");
+    if (nb < 1) return fail(RTE_E_ARG, "rte_solve_media: nb < 1");
+    int rc;
+    if ((rc = check_params(params))) return rc;
+    for (int i = 0; i < nb; ++i) {
+        if (!media[i]) return fail(RTE_E_ARG, "media[%d] is NULL", i);
+        if (media[i]->ctx != ctx) return fail(RTE_E_ARG, "media[%d] belongs to another context", i);
+        const rte_grid &a = media[i]->grid, &b = media[0]->grid;
+        if (a.nx != b.nx || a.ny != b.ny || a.nz != b.nz || a.xmin != b.xmin || a.xmax != b.xmax ||
+            a.ymin != b.ymin || a.ymax != b.ymax || a.zmin != b.zmin || a.zmax != b.zmax)
+            return fail(RTE_E_ARG, "media[%d] has a different grid than media[0]", i);
+        if (!std::isfinite(beams[i].pos_x) || !std::isfinite(beams[i].pos_y) || !(beams[i].diam >= 0.0))
+            return fail(RTE_E_ARG, "beam %d invalid", i);
+    }
+    if ((rc = set_device(ctx))) return rc;
+    switch (params->precision) {
+    case RTE_PREC_FP64: return run_batch<double, double>(ctx, nb, media, beams, params, exitance, stats);
+    case RTE_PREC_MIXED: return run_batch<float, double>(ctx, nb, media, beams, params, exitance, stats);
+    default: return run_batch<float, float>(ctx, nb, media, beams, params, exitance, stats);
+    }
+}
+
+extern "C" int rte_solve(rte_ctx *ctx, const rte_grid *grid, const rte_optics *optics,
+                         const rte_beam *beam, const rte_params *params, double *exitance,
+                         rte_stats *stats)
+{
+    rte_medium *m = nullptr;
+    int rc = rte_medium_create(ctx, grid, optics, &m);
+    if (rc) return rc;
+    rc = rte_solve_media(ctx, 1, &m, beam, params, exitance, stats);
+    std::string keep = g_err;
+    rte_medium_destroy(m);
+    g_err = keep;
+    return rc;
+}
+
+extern "C" long long rte_solve_footprint(const rte_grid *grid, const rte_params *params, int path_mode)
+{
+    if (!grid || !params || params->L < 0 || params->L > RTE_MAX_L) return -1;
+    const GridDev<float> g = make_grid<float>(*grid);
+    const long long K = (params->L + 1) * (params->L + 1);
+    const long long es = params->precision == RTE_PREC_FP64 ? 8 : 4;
+    const long long N = g.N;
+    const int nch = pick_chunks(g.nblk, 1, g.N);
+    long long bytes = N * 8 * 3 + N * 4 * 2;             // maps
+    bytes += N * K * es * (1 + nch);                      // W + partials
+    bytes += (long long)grid->nx * grid->ny * 3 * 8 + 64; // pixel buffers
+    if (path_mode == RTE_PATH_CACHED) bytes += N * (long long)g.npad * es;
+    return bytes;
+}
+
+// ---------------------------------------------------------------------------
+// operators
+// ---------------------------------------------------------------------------
+template <typename T, typename TA>
+static int propagate_impl(rte_ctx *ctx, rte_medium *m, int L, double prop_thresh, int path,
+                          const double *W, double *out)
+{
+    const rte_grid &grid = m->grid;
+    const int K = (L + 1) * (L + 1);
+    const GridDev<T> g = make_grid<T>(grid);
+    const int N = g.N;
+    const double dx = (grid.xmax - grid.xmin) / grid.nx, dy = (grid.ymax - grid.ymin) / grid.ny,
+                 dz = (grid.zmax - grid.zmin) / grid.nz;
+    const double thr = prop_thresh > 0 ? prop_thresh : -1.0;
+    hipStream_t st = ctx->stream;
+    MomentSet *mom = nullptr;
+    int rc;
+    if ((rc = get_moments(ctx, L, dx, dy, dz, &mom))) return rc;
+    if (path == RTE_PATH_CACHED && (rc = ensure_ktab<T>(m, thr))) return rc;
+    const int nch0 = pick_chunks(g.nblk, 1, N);
+    const int clen = (N + nch0 - 1) / nch0;
+    const int nch = (N + clen - 1) / clen;
+    DevBuf dIn, dW, dPart, dAW, dOut, dSv;
+    HIP_TRY(dIn.alloc((size_t)K * N * 8));
+    HIP_TRY(dW.alloc((size_t)K * N * sizeof(T)));
+    HIP_TRY(dPart.alloc((size_t)nch * N * K * sizeof(T)));
+    HIP_TRY(dAW.alloc((size_t)K * N * sizeof(TA)));
+    HIP_TRY(dOut.alloc((size_t)K * N * 8));
+    HIP_TRY(dSv.alloc(sizeof(SolveDev<T, TA>)));
+    HIP_TRY(hipMemcpyAsync(dIn.p, W, (size_t)K * N * 8, hipMemcpyHostToDevice, st));
+    const Scales sc = make_scales(L);
+    HIP_TRY(launch_to_voxel_major<T>(K, N, sc, dIn.as<double>(), dW.as<T>(), st));
+    SolveDev<T, TA> s;
+    std::memset(&s, 0, sizeof s);
+    s.W = dW.as<T>();
+    s.part = dPart.as<T>();
+    s.ktab = sizeof(T) == 4 ? (const T *)m->ktab_f.p : (const T *)m->ktab_d.p;
+    s.mut = sizeof(T) == 4 ? (const T *)m->mut_f.p : (const T *)m->mut_d.p;
+    s.mus = sizeof(T) == 4 ? (const T *)m->mus_f.p : (const T *)m->mus_d.p;
+    s.g = T(m->g);
+    HIP_TRY(hipMemcpyAsync(dSv.p, &s, sizeof s, hipMemcpyHostToDevice, st));
+    PropArgs<T> pa;
+    pa.g = g;
+    pa.sh = make_sh_const<T>();
+    pa.nchunk = nch;
+    pa.chunk_len = clen;
+    pa.thr2 = thr > 0 ? T(thr * thr) : T(sizeof(T) == 4 ? 1e30 : 1e300);
+    HIP_TRY(launch_prop<T, TA>(L, path, pa, dSv.as<SolveDev<T, TA>>(), 1, st));
+    CombArgs<T, TA> ca;
+    std::memset(&ca, 0, sizeof ca);
+    ca.g = g;
+    ca.nchunk = nch;
+    ca.op = 1;
+    ca.mom = sizeof(TA) == 4 ? (const TA *)mom->mf.p : (const TA *)mom->md.p;
+    ca.self_scale = TA((M_PI / kAngRes) * (2.0 * M_PI / kAngRes) / (dx * dy * dz));
+    ca.dV = TA(dx * dy * dz);
+    ca.aw_out = dAW.as<TA>();
+    for (int a = 0; a < kMaxK; ++a) ca.nsc[a] = TA(sc.n[a]);
+    HIP_TRY(launch_combine<T, TA>(L, ca, dSv.as<SolveDev<T, TA>>(), 1, st));
+    HIP_TRY(launch_to_coef_major<TA>(K, N, unit_scales(), dAW.as<TA>(), dOut.as<double>(), st));
+    HIP_TRY(hipMemcpyAsync(out, dOut.p, (size_t)K * N * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    // the reference's A carries 1/c_m (CPU package/src/Kernels_selfsub.c:948,1046)
+    const double cm = m->c / m->n;
+    for (size_t i = 0; i < (size_t)K * N; ++i) out[i] /= cm;
+    return RTE_OK;
+}
+
+extern "C" int rte_propagate(rte_ctx *ctx, const rte_grid *grid, const rte_optics *optics, int L,
+                             double prop_thresh, int precision, int path_mode, const double *W,
+                             double *out)
+{
+    if (!ctx || !W || !out) return fail(RTE_E_ARG, "rte_propagate: NULL argument");
+    if (L < 0 || L > RTE_MAX_L) return fail(RTE_E_UNSUPP, "L=%d outside 0..%d", L, RTE_MAX_L);
+    if (path_mode != RTE_PATH_CACHED && path_mode != RTE_PATH_MARCH)
+        return fail(RTE_E_ARG, "unknown path_mode %d", path_mode);
+    rte_medium *m = nullptr;
+    int rc = rte_medium_create(ctx, grid, optics, &m);
+    if (rc) return rc;
+    switch (precision) {
+    case RTE_PREC_FP64: rc = propagate_impl<double, double>(ctx, m, L, prop_thresh, path_mode, W, out); break;
+    case RTE_PREC_MIXED: rc = propagate_impl<float, double>(ctx, m, L, prop_thresh, path_mode, W, out); break;
+    case RTE_PREC_FP32: rc = propagate_impl<float, float>(ctx, m, L, prop_thresh, path_mode, W, out); break;
+    default: rc = fail(RTE_E_ARG, "unknown precision %d", precision);
+    }
+    std::string keep = g_err;
+    rte_medium_destroy(m);
+    g_err = keep;
+    return rc;
+}
+
+extern "C" int rte_scatter(rte_ctx *ctx, const rte_grid *grid, const rte_optics *optics, int L, double *W)
+{
+    if (!ctx || !W) return fail(RTE_E_ARG, "rte_scatter: NULL argument");
+    if (L < 0 || L > RTE_MAX_L) return fail(RTE_E_UNSUPP, "L=%d outside 0..%d", L, RTE_MAX_L);
+    int rc;
+    if ((rc = check_grid(grid)) || (rc = check_optics(grid, optics)) || (rc = set_device(ctx))) return rc;
+    const int N = grid->nx * grid->ny * grid->nz, K = (L + 1) * (L + 1);
+    DevBuf dW, dmus;
+    HIP_TRY(dW.alloc((size_t)K * N * 8));
+    HIP_TRY(dmus.alloc((size_t)N * 8));
+    HIP_TRY(hipMemcpyAsync(dW.p, W, (size_t)K * N * 8, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(dmus.p, optics->mu_s, (size_t)N * 8, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(launch_scatter_op(L, N, optics->g, dmus.as<double>(), dW.as<double>(), ctx->stream));
+    HIP_TRY(hipMemcpyAsync(W, dW.p, (size_t)K * N * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    // the reference's D carries c_m (CPU package/src/Kernels_selfsub.c:42)
+    const double cm = optics->c / optics->n;
+    for (size_t i = 0; i < (size_t)K * N; ++i) W[i] *= cm;
+    return RTE_OK;
+}
+
+extern "C" int rte_source(rte_ctx *ctx, const rte_grid *grid, const rte_optics *optics,
+                          const rte_beam *beam, int L, double *s0, double *ballistic, int *ncols)
+{
+    if (!ctx || !beam || !s0) return fail(RTE_E_ARG, "rte_source: NULL argument");
+    if (L < 0 || L > RTE_MAX_L) return fail(RTE_E_UNSUPP, "L=%d outside 0..%d", L, RTE_MAX_L);
+    rte_medium *m = nullptr;
+    int rc = rte_medium_create(ctx, grid, optics, &m);
+    if (rc) return rc;
+    const int N = m->N, K = (L + 1) * (L + 1), P = grid->nx * grid->ny;
+    const double dx = (grid->xmax - grid->xmin) / grid->nx, dy = (grid->ymax - grid->ymin) / grid->ny,
+                 dz = (grid->zmax - grid->zmin) / grid->nz;
+    const double half = (beam->diam + 1.0e-16) / 2.0;
+    int cols = 0;
+    for (int j = 0; j < grid->ny; ++j)
+        for (int i = 0; i < grid->nx; ++i) {
+            const double xc = (double)i * dx + grid->xmin + dx / 2.0;
+            const double yc = (double)j * dy + grid->ymin + dy / 2.0;
+            if (std::sqrt((xc - beam->pos_x) * (xc - beam->pos_x) + (yc - beam->pos_y) * (yc - beam->pos_y)) <=
+                half)
+                ++cols;
+        }
+    if (ncols) *ncols = cols;
+    auto run = [&]() -> int {
+        if (cols == 0) return fail(RTE_E_ARG, "beam hits no voxel column");
+        hipStream_t st = ctx->stream;
+        DevBuf dW, dpix, dred, dsv, dbeam, dout;
+        HIP_TRY(dW.alloc((size_t)K * N * 8));
+        HIP_TRY(dpix.alloc((size_t)3 * P * 8));
+        HIP_TRY(dred.alloc(16));
+        HIP_TRY(dsv.alloc(sizeof(SolveDev<double, double>)));
+        HIP_TRY(dbeam.alloc(sizeof(BeamDev)));
+        HIP_TRY(dout.alloc((size_t)K * N * 8));
+        HIP_TRY(launch_fill<double>(dW.as<double>(), (size_t)K * N, 0.0, st));
+        SolveDev<double, double> s;
+        std::memset(&s, 0, sizeof s);
+        s.W = dW.as<double>();
+        s.g = m->g;
+        s.dE = dpix.as<double>();
+        s.bsum = s.dE + P;
+        s.E = s.dE + 2 * P;
+        s.red = dred.as<double>();
+        BeamDev b{beam->pos_x, beam->pos_y, half, beam->mag / cols, m->mua_d.as<double>(), m->mus_d.as<double>()};
+        HIP_TRY(hipMemcpyAsync(dsv.p, &s, sizeof s, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(dbeam.p, &b, sizeof b, hipMemcpyHostToDevice, st));
+        const GridDev<double> g = make_grid<double>(*grid);
+        const Scales sc = make_scales(L);
+        HIP_TRY(launch_source<double, double>(L, g, dx, dy, dz, grid->xmin, grid->ymin, sc, dbeam.as<BeamDev>(),
+                                              dsv.as<SolveDev<double, double>>(), 1, st));
+        HIP_TRY(launch_to_coef_major<double>(K, N, sc, dW.as<double>(), dout.as<double>(), st));
+        HIP_TRY(hipMemcpyAsync(s0, dout.p, (size_t)K * N * 8, hipMemcpyDeviceToHost, st));
+        if (ballistic)
+            HIP_TRY(hipMemcpyAsync(ballistic, s.E, (size_t)P * 8, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        return RTE_OK;
+    };
+    rc = run();
+    std::string keep = g_err;
+    rte_medium_destroy(m);
+    g_err = keep;
+    return rc;
+}
+
+extern "C" int rte_exitance(rte_ctx *ctx, const rte_grid *grid, int L, int exit_quad, double c_m,
+                            const double *w, double *img)
+{
+    if (!ctx || !w || !img) return fail(RTE_E_ARG, "rte_exitance: NULL argument");
+    if (!(c_m > 0.0)) return fail(RTE_E_ARG, "rte_exitance: c_m must be > 0");
+    if (L < 0 || L > RTE_MAX_L) return fail(RTE_E_UNSUPP, "L=%d outside 0..%d", L, RTE_MAX_L);
+    if (exit_quad != RTE_QUAD_REF && exit_quad != RTE_QUAD_EXACT) return fail(RTE_E_ARG, "unknown exit_quad");
+    int rc;
+    if ((rc = check_grid(grid)) || (rc = set_device(ctx))) return rc;
+    const int N = grid->nx * grid->ny * grid->nz, K = (L + 1) * (L + 1), P = grid->nx * grid->ny;
+    const double dx = (grid->xmax - grid->xmin) / grid->nx, dy = (grid->ymax - grid->ymin) / grid->ny;
+    double Q[kMaxL + 1], coef[kMaxL + 1];
+    quad_weights(L, exit_quad, Q);
+    for (int l = 0; l <= L; ++l)
+        coef[l] = 2.0 * M_PI * dx * dy * c_m / std::sqrt(4.0 * M_PI) * std::sqrt(2.0 * l + 1.0) * Q[l];
+    DevBuf dw, dc, dimg;
+    HIP_TRY(dw.alloc((size_t)K * N * 8));
+    HIP_TRY(dc.alloc(sizeof coef));
+    HIP_TRY(dimg.alloc((size_t)P * 8));
+    HIP_TRY(hipMemcpyAsync(dw.p, w, (size_t)K * N * 8, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(dc.p, coef, sizeof coef, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(launch_exitance_op(L, grid->nx, grid->ny, grid->nz, dc.as<double>(), dw.as<double>(),
+                               dimg.as<double>(), ctx->stream));
+    HIP_TRY(hipMemcpyAsync(img, dimg.p, (size_t)P * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return RTE_OK;
+}
+
+extern "C" int rte_path_integrals(rte_ctx *ctx, const rte_grid *grid, const rte_optics *optics, int npairs,
+                                  const int *dest, const int *src, double *tau)
+{
+    if (!ctx || !dest || !src || !tau || npairs < 0) return fail(RTE_E_ARG, "rte_path_integrals: bad argument");
+    rte_medium *m = nullptr;
+    int rc = rte_medium_create(ctx, grid, optics, &m);
+    if (rc) return rc;
+    for (int i = 0; i < npairs; ++i)
+        if (dest[i] < 0 || dest[i] >= m->N || src[i] < 0 || src[i] >= m->N) {
+            rte_medium_destroy(m);
+            return fail(RTE_E_ARG, "pair %d outside the grid", i);
+        }
+    auto run = [&]() -> int {
+        if (npairs == 0) return RTE_OK;
+        DevBuf dd, ds, dt;
+        HIP_TRY(dd.alloc((size_t)npairs * 4));
+        HIP_TRY(ds.alloc((size_t)npairs * 4));
+        HIP_TRY(dt.alloc((size_t)npairs * 8));
+        HIP_TRY(hipMemcpyAsync(dd.p, dest, (size_t)npairs * 4, hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(ds.p, src, (size_t)npairs * 4, hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(launch_tau_list(make_grid<float>(*grid), m->mut_f.as<float>(), npairs, dd.as<int>(), ds.as<int>(),
+                                dt.as<double>(), ctx->stream));
+        HIP_TRY(hipMemcpyAsync(tau, dt.p, (size_t)npairs * 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        return RTE_OK;
+    };
+    rc = run();
+    std::string keep = g_err;
+    rte_medium_destroy(m);
+    g_err = keep;
+    return rc;
+}

@@ -1,0 +1,123 @@
+// rte_internal.hpp -- structures shared by the host library and the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace rte {
+
+constexpr int kMaxL = 7;
+constexpr int kMaxK = (kMaxL + 1) * (kMaxL + 1);
+constexpr int kNMom = 16;      // moment-series terms of the self-voxel block
+constexpr int kAngRes = 200;   // self-block quadrature grid (CPU package/src/Kernels_selfsub.c:737)
+constexpr int kNOmega = kAngRes * kAngRes;
+constexpr int kBrick = 4;      // destination bricks of 4x4x4 voxels = one wave64
+constexpr int kWave = 64;
+
+// Geometry in device form.  Voxel v = x + nx*(y + ny*z); destination bricks
+// are numbered bx fastest; slot = brick*64 + lane, lane = lx + 4*(ly + 4*lz).
+template <typename T>
+struct GridDev {
+    int nx, ny, nz, N;
+    int bx, by, bz, nblk;  // brick counts per axis and total
+    int npad;              // nblk * 64
+    T dx, dy, dz, dV;
+};
+
+// Real SH through monic recurrences.  With P_l^m = (1-z^2)^{m/2} Q_l^m (Condon-
+// Shortley phase in Q), Q_l^m = c_lm Pt_l^m where
+//   Pt_m^m = 1,  Pt_{m+1}^m = z,  Pt_l^m = z Pt_{l-1}^m - gamma[l][m] Pt_{l-2}^m,
+//   gamma_lm = (l+m-1)(l-m-1) / ((2l-1)(2l-3)),
+// so that R_l0 = n_l0 Pt_l^0, R_lm = n_lm Pt_l^m Re((x+iy)^m),
+// R_l,-m = n_lm Pt_l^m Im((x+iy)^m), n_lm = s_m N_lm c_lm (s_m = sqrt2 for m>0).
+// Device fields hold the scaled coefficients W'_a = n_a W_a, so a pair costs
+// no multiplication by n; partial sums come out scaled by 1/n_a.
+template <typename T>
+struct ShConst {
+    T gamma[kMaxL + 1][kMaxL + 1];
+};
+
+// Per-solve device state of one Neumann series.
+template <typename T, typename TA>
+struct SolveDev {
+    T *W;              // [N][K] current term (input of the propagation)
+    T *part;           // [nchunk][N][K] partial pair sums
+    const T *ktab;     // [N][npad] e^{-tau}/r^2 per (source, destination slot), cached mode
+    const T *mut;      // [N] mu_t
+    const T *mus;      // [N] mu_s
+    T g;               // anisotropy
+    TA *dE;            // [nx*ny] exitance increment of this term
+    TA *bsum;          // [nx*ny] per-pixel bottom sums of the next term (stop rule)
+    TA *E;             // [nx*ny] accumulated exitance (ballistic included)
+    TA *red;           // [2] reduced (sum dE, sum bsum) of this term
+    int mode;          // 0 term, 1 init, 2 final (see k_bottom / k_combine)
+};
+
+template <typename T>
+struct PropArgs {
+    GridDev<T> g;
+    ShConst<T> sh;
+    int nchunk, chunk_len;
+    T thr2;  // prop_thresh^2; pairs with r^2 >= thr2 are dropped
+};
+
+// combine launch: A.op = 1 writes A W to aw_out; otherwise the per-solve mode
+// decides (0 term: scatter + exitance + stop sums, 2 final: exitance only)
+template <typename T, typename TA>
+struct CombArgs {
+    GridDev<T> g;
+    int nchunk;
+    int op;
+    const TA *mom;        // [kNMom][K][K] self-block moments
+    TA self_scale;        // dtheta dphi / dV (S's 1/dV; the result is scaled by dV)
+    TA dV;
+    TA coefE[kMaxL + 1];  // exitance weight of w_l0
+    TA wsum[kMaxK];       // stop-rule weight of coefficient a
+    TA nsc[kMaxK];        // n_a of the scaled coefficients
+    TA *aw_out;           // op = 1
+};
+
+// n_a of the scaled coefficients, by value
+struct Scales {
+    double n[kMaxK];
+};
+
+struct BeamDev {
+    double pos_x, pos_y, half_diam, fact;  // fact = power per beam column
+    const double *mua, *mus;               // FP64 maps
+};
+
+// launchers (rte_kernels.hip)
+template <typename T>
+hipError_t launch_fill(T *p, size_t n, T v, hipStream_t st);
+template <typename T, typename TA>
+hipError_t launch_prop(int L, int path, const PropArgs<T> &A, const SolveDev<T, TA> *sv, int nsolve,
+                       hipStream_t st);
+template <typename T>
+hipError_t launch_ktab(const GridDev<T> &g, int nchunk, int chunk_len, T thr2, const T *mut, T *ktab,
+                       hipStream_t st);
+template <typename T, typename TA>
+hipError_t launch_combine(int L, const CombArgs<T, TA> &A, const SolveDev<T, TA> *sv, int nsolve,
+                          hipStream_t st);
+template <typename T, typename TA>
+hipError_t launch_bottom(int P, const SolveDev<T, TA> *sv, int nsolve, hipStream_t st);
+template <typename T, typename TA>
+hipError_t launch_source(int L, const GridDev<T> &g, double dx, double dy, double dz, double xmin,
+                         double ymin, const Scales &sc, const BeamDev *beams, const SolveDev<T, TA> *sv,
+                         int nsolve, hipStream_t st);
+hipError_t launch_moments(int K, const double *R, const double *sn, const double *beta, double *M,
+                          hipStream_t st);
+// in: coefficient-major double [K][N] -> out: voxel-major T [N][K], times sc.n[a]
+template <typename T>
+hipError_t launch_to_voxel_major(int K, int N, const Scales &sc, const double *in, T *out, hipStream_t st);
+// in: voxel-major TA [N][K] -> out: coefficient-major double [K][N], divided by sc.n[a]
+template <typename TA>
+hipError_t launch_to_coef_major(int K, int N, const Scales &sc, const TA *in, double *out, hipStream_t st);
+hipError_t launch_scatter_op(int L, int N, double g, const double *mus, double *W, hipStream_t st);
+hipError_t launch_exitance_op(int L, int nx, int ny, int nz, const double *coefE, const double *w,
+                              double *img, hipStream_t st);
+hipError_t launch_tau_list(const GridDev<float> &g, const float *mut, int np, const int *dst,
+                           const int *src, double *tau, hipStream_t st);
+
+}  // namespace rte

@@ -1,0 +1,714 @@
+// rte_kernels.hip -- CDNA4 (gfx950) kernels of the Neumann-series RTE solve.
+//
+// Propagation A (the hot path, CPU package/src/Kernels_selfsub.c:450-538 and
+// GPU package/src/kernels.cu:286-364) is evaluated pair by pair in the real SH
+// basis, factored through the direction of the pair:
+//   (A W)_a(d) = dV [ sum_{s != d} k(d,s) R_a(u) (R(u) . W(s)) + (S(mu_t(d)) W(d))_a ],
+//   k(d,s) = e^{-tau(d,s)} / r^2,   u = (r_d - r_s)/r,
+// i.e. 2K FMAs per pair instead of the reference's K^2 complex MACs.
+//
+// Work decomposition: one wave64 owns a 4x4x4 brick of destination voxels (one
+// per lane) and walks a contiguous chunk of source voxels in lockstep, so the
+// source's coefficients W(s) are wave-uniform (scalar loads into SGPRs) and
+// every lane is a valid pair; R(u) is evaluated per lane with the Cartesian
+// recurrences, fused into the dot product and the axpy.  The path integral
+// either comes from a per-medium table (cached mode, written once by
+// k_ktab) or is marched per pair in every term (march mode, the reference's
+// structure, GPU package/src/kernels.cu:49-145).  Partial sums per source
+// chunk are combined in a fixed order by k_combine, so results do not depend
+// on how many solves share a launch.
+#include "rte_internal.hpp"
+
+namespace rte {
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__device__ __forceinline__ T dev_rsqrt(T x);
+template <>
+__device__ __forceinline__ float dev_rsqrt<float>(float x) { return __frsqrt_rn(x); }
+template <>
+__device__ __forceinline__ double dev_rsqrt<double>(double x) { return 1.0 / sqrt(x); }
+
+template <typename T>
+__device__ __forceinline__ T dev_exp(T x);
+template <>
+__device__ __forceinline__ float dev_exp<float>(float x) { return expf(x); }
+template <>
+__device__ __forceinline__ double dev_exp<double>(double x) { return exp(x); }
+
+template <typename T>
+__device__ __forceinline__ T big_value();
+template <>
+__device__ __forceinline__ float big_value<float>() { return 1.0e30f; }
+template <>
+__device__ __forceinline__ double big_value<double>() { return 1.0e300; }
+
+template <typename T>
+__global__ void k_fill(T *__restrict__ p, size_t n, T v)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Siddon path integral between the centres of source s = (si,sj,sk) and
+// destination d = s + (ix,iy,iz):  tau = r * sum_v mu_t(v) dalpha(v).
+// Crossings along an axis with n steps sit at alpha = (j + 1/2)/n; ties go to
+// z, then y, then x; the destination contributes the half step 0.5/n of the
+// axis it was entered through (GPU package/src/kernels.cu:49-145).
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T march_tau(const __attribute__((address_space(1))) T *mut, int v, int ix, int iy, int iz,
+                                       int nx, int nxy, T r)
+{
+    const int n0 = abs(ix), n1 = abs(iy), n2 = abs(iz);
+    const T BIG = big_value<T>();
+    const T inc0 = n0 ? T(1) / T(n0) : BIG;
+    const T inc1 = n1 ? T(1) / T(n1) : BIG;
+    const T inc2 = n2 ? T(1) / T(n2) : BIG;
+    T a0 = T(0.5) * inc0, a1 = T(0.5) * inc1, a2 = T(0.5) * inc2;
+    const int st0 = ix > 0 ? 1 : -1, st1 = iy > 0 ? nx : -nx, st2 = iz > 0 ? nxy : -nxy;
+    const int nsteps = n0 + n1 + n2;
+    T prev = T(0), acc = T(0), last = T(0);
+    for (int t = 0; t < nsteps; ++t) {
+        const bool pz = (a2 <= a0) && (a2 <= a1);
+        const bool py = !pz && (a1 <= a0);
+        const T a = pz ? a2 : (py ? a1 : a0);
+        acc += mut[v] * (a - prev);
+        prev = a;
+        v += pz ? st2 : (py ? st1 : st0);
+        last = pz ? inc2 : (py ? inc1 : inc0);
+        a2 = pz ? a2 + inc2 : a2;
+        a1 = py ? a1 + inc1 : a1;
+        a0 = (!pz && !py) ? a0 + inc0 : a0;
+    }
+    acc += mut[v] * T(0.5) * last;
+    return acc * r;
+}
+
+// ---------------------------------------------------------------------------
+// One (destination, source) pair in the factored real-SH form, on scaled
+// coefficients (see ShConst):  p = sum_a Rt_a(u) w'_a ;  acc' += (kf p) Rt(u)
+// with Rt_l0 = Pt_l^0, Rt_lm = Pt_l^m C_m, Rt_l,-m = Pt_l^m S_m,
+// C_m + i S_m = (x + i y)^m.
+// ---------------------------------------------------------------------------
+template <typename T, int L>
+__device__ __forceinline__ void pair_accumulate(T (&acc)[(L + 1) * (L + 1)],
+                                                const T (&w)[(L + 1) * (L + 1)], T kf, T x, T y,
+                                                T z, const ShConst<T> &sh)
+{
+    T C[L + 1], S[L + 1];
+    C[0] = T(1);
+    S[0] = T(0);
+    if constexpr (L >= 1) {
+        C[1] = x;
+        S[1] = y;
+    }
+#pragma unroll
+    for (int m = 2; m <= L; ++m) {
+        C[m] = C[m - 1] * x - S[m - 1] * y;
+        S[m] = S[m - 1] * x + C[m - 1] * y;
+    }
+    T P[L + 1][L + 1];  // only l >= m used
+#pragma unroll
+    for (int m = 0; m <= L; ++m) {
+        P[m][m] = T(1);
+        if (m + 1 <= L) P[m + 1][m] = z;
+#pragma unroll
+        for (int l = m + 2; l <= L; ++l) P[l][m] = z * P[l - 1][m] - sh.gamma[l][m] * P[l - 2][m];
+    }
+    T p = w[0];
+#pragma unroll
+    for (int l = 1; l <= L; ++l) p += P[l][0] * w[l * l + l];
+#pragma unroll
+    for (int m = 1; m <= L; ++m) {
+        T A = w[m * m + m + m], B = w[m * m + m - m];
+#pragma unroll
+        for (int l = m + 1; l <= L; ++l) {
+            A += P[l][m] * w[l * l + l + m];
+            B += P[l][m] * w[l * l + l - m];
+        }
+        p += C[m] * A + S[m] * B;
+    }
+    const T t = kf * p;
+    acc[0] += t;
+#pragma unroll
+    for (int l = 1; l <= L; ++l) acc[l * l + l] += t * P[l][0];
+#pragma unroll
+    for (int m = 1; m <= L; ++m) {
+        const T u = t * C[m], vv = t * S[m];
+        acc[m * m + m + m] += u;
+        acc[m * m + m - m] += vv;
+#pragma unroll
+        for (int l = m + 1; l <= L; ++l) {
+            acc[l * l + l + m] += u * P[l][m];
+            acc[l * l + l - m] += vv * P[l][m];
+        }
+    }
+}
+
+// address-space views: wave-uniform reads through the scalar cache, per-lane
+// reads as global (not flat) loads
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T *;
+template <typename T>
+using gptr = const __attribute__((address_space(1))) T *;
+template <typename T>
+__device__ __forceinline__ cptr<T> as_const(const T *p) { return (cptr<T>)(p); }
+template <typename T>
+__device__ __forceinline__ gptr<T> as_global(const T *p) { return (gptr<T>)(p); }
+
+// destination of a lane: brick blk, lane -> (di, dj, dk)
+template <typename T>
+__device__ __forceinline__ void lane_dest(const GridDev<T> &g, int blk, int lane, int &di, int &dj,
+                                          int &dk)
+{
+    const int bxi = blk % g.bx, byi = (blk / g.bx) % g.by, bzi = blk / (g.bx * g.by);
+    di = bxi * kBrick + (lane & 3);
+    dj = byi * kBrick + ((lane >> 2) & 3);
+    dk = bzi * kBrick + (lane >> 4);
+}
+
+// ---------------------------------------------------------------------------
+// Propagation: partial pair sums over one chunk of sources.
+//   grid = (nblk, nchunk, nsolve), block = 64
+// PATH 0: kf from the per-medium table; PATH 1: marched per pair.
+// ---------------------------------------------------------------------------
+template <typename T, typename TA, int L, int PATH>
+__global__ __launch_bounds__(kWave) void k_prop(PropArgs<T> A, const SolveDev<T, TA> *__restrict__ sv)
+{
+    constexpr int K = (L + 1) * (L + 1);
+    const GridDev<T> &g = A.g;
+    const int lane = threadIdx.x, blk = blockIdx.x, chunk = blockIdx.y;
+    const SolveDev<T, TA> S = sv[blockIdx.z];
+    int di, dj, dk;
+    lane_dest(g, blk, lane, di, dj, dk);
+    const bool valid = di < g.nx && dj < g.ny && dk < g.nz;
+    // keep marching lanes inside the grid; their kf is zeroed below
+    const int cdi = min(di, g.nx - 1), cdj = min(dj, g.ny - 1), cdk = min(dk, g.nz - 1);
+    const int slot = blk * kWave + lane;
+    const int s0 = chunk * A.chunk_len;
+    const int s1 = min(g.N, s0 + A.chunk_len);
+    const int nxy = g.nx * g.ny;
+
+    T acc[K];
+#pragma unroll
+    for (int a = 0; a < K; ++a) acc[a] = T(0);
+
+    int si = s0 % g.nx, sj = (s0 / g.nx) % g.ny, sk = s0 / nxy;
+    const cptr<T> Wsrc = as_const<T>(S.W);
+    const gptr<T> ktab = as_global<T>(S.ktab);
+    for (int s = s0; s < s1; ++s) {
+        T w[K];
+#pragma unroll
+        for (int a = 0; a < K; ++a) w[a] = Wsrc[(size_t)s * K + a];
+        const int ix = cdi - si, iy = cdj - sj, iz = cdk - sk;
+        const T ddx = T(ix) * g.dx, ddy = T(iy) * g.dy, ddz = T(iz) * g.dz;
+        T r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+        const bool self = (ix == 0) && (iy == 0) && (iz == 0);
+        T kf;
+        if constexpr (PATH == 0) {
+            kf = ktab[(size_t)s * g.npad + slot];
+        } else {
+            const bool use = valid && !self && (r2 < A.thr2);
+            const T r = sqrt(self ? T(1) : r2);
+            const T tau = use ? march_tau<T>(as_global<T>(S.mut), s, ix, iy, iz, g.nx, nxy, r) : T(0);
+            kf = use ? dev_exp<T>(-tau) / r2 : T(0);
+        }
+        r2 = self ? T(1) : r2;
+        const T rinv = dev_rsqrt<T>(r2);
+        pair_accumulate<T, L>(acc, w, kf, ddx * rinv, ddy * rinv, ddz * rinv, A.sh);
+        if (++si == g.nx) {
+            si = 0;
+            if (++sj == g.ny) {
+                sj = 0;
+                ++sk;
+            }
+        }
+    }
+    if (valid) {
+        const int v = di + g.nx * (dj + g.ny * dk);
+        T *__restrict__ out = S.part + ((size_t)chunk * g.N + v) * K;
+#pragma unroll
+        for (int a = 0; a < K; ++a) out[a] = acc[a];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-medium table kf(d, s) = e^{-tau(d,s)}/r^2 (0 for d == s, r >= thresh and
+// padding lanes), layout [s][slot].  grid = (nblk, nchunk), block = 64.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kWave) void k_ktab(GridDev<T> g, int chunk_len, T thr2,
+                                               const T *__restrict__ mut, T *__restrict__ ktab)
+{
+    const int lane = threadIdx.x, blk = blockIdx.x, chunk = blockIdx.y;
+    int di, dj, dk;
+    lane_dest(g, blk, lane, di, dj, dk);
+    const bool valid = di < g.nx && dj < g.ny && dk < g.nz;
+    const int cdi = min(di, g.nx - 1), cdj = min(dj, g.ny - 1), cdk = min(dk, g.nz - 1);
+    const int slot = blk * kWave + lane;
+    const int s0 = chunk * chunk_len;
+    const int s1 = min(g.N, s0 + chunk_len);
+    const int nxy = g.nx * g.ny;
+    int si = s0 % g.nx, sj = (s0 / g.nx) % g.ny, sk = s0 / nxy;
+    for (int s = s0; s < s1; ++s) {
+        const int ix = cdi - si, iy = cdj - sj, iz = cdk - sk;
+        const T ddx = T(ix) * g.dx, ddy = T(iy) * g.dy, ddz = T(iz) * g.dz;
+        const T r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+        const bool self = (ix == 0) && (iy == 0) && (iz == 0);
+        const bool use = valid && !self && (r2 < thr2);
+        const T r = sqrt(self ? T(1) : r2);
+        const T tau = use ? march_tau<T>(as_global<T>(mut), s, ix, iy, iz, g.nx, nxy, r) : T(0);
+        ktab[(size_t)s * g.npad + slot] = use ? dev_exp<T>(-tau) / r2 : T(0);
+        if (++si == g.nx) {
+            si = 0;
+            if (++sj == g.ny) {
+                sj = 0;
+                ++sk;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Combine: chunk partials (fixed order) + self block, times dV, then
+//   operator launch (A.op = 1): aw_out <- A W (TA, [v][K])
+//   per-solve mode 0 (term):  W <- g^l mu_s A W, dE = exitance(A W), bsum = stop-rule sums of W
+//   per-solve mode 2 (final): dE only
+// grid = (ceil(N/64), nsolve), block = 64.
+// ---------------------------------------------------------------------------
+template <typename T, typename TA, int L>
+__global__ __launch_bounds__(kWave) void k_combine(CombArgs<T, TA> A, const SolveDev<T, TA> *__restrict__ sv)
+{
+    constexpr int K = (L + 1) * (L + 1);
+    const GridDev<T> &g = A.g;
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const SolveDev<T, TA> S = sv[blockIdx.y];
+    if (v >= g.N) return;
+    TA acc[K], w[K];
+#pragma unroll
+    for (int a = 0; a < K; ++a) {
+        acc[a] = TA(0);
+        w[a] = TA(S.W[(size_t)v * K + a]) / A.nsc[a];
+    }
+    for (int c = 0; c < A.nchunk; ++c) {
+        const T *__restrict__ p = S.part + ((size_t)c * g.N + v) * K;
+#pragma unroll
+        for (int a = 0; a < K; ++a) acc[a] += TA(p[a]);
+    }
+#pragma unroll
+    for (int a = 0; a < K; ++a) acc[a] *= A.nsc[a];
+    // self block: sum_k coef_k (M_k w), coef_k = (-mu)^{k-1}/k!
+    const TA mu = TA(S.mut[v]);
+    TA coef = TA(1);
+    for (int k = 0; k < kNMom; ++k) {
+        if (k > 0) coef = coef * (-mu) / TA(k + 1);
+        const TA *__restrict__ M = A.mom + (size_t)k * K * K;
+#pragma unroll
+        for (int a = 0; a < K; ++a) {
+            TA s = TA(0);
+#pragma unroll
+            for (int b = 0; b < K; ++b) s += M[a * K + b] * w[b];
+            acc[a] += A.self_scale * coef * s;
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < K; ++a) acc[a] *= A.dV;
+
+    const int zk = v / (g.nx * g.ny);
+    const int pix = v - zk * g.nx * g.ny;
+    const bool bottom = (zk == g.nz - 1);
+    if (A.op) {
+#pragma unroll
+        for (int a = 0; a < K; ++a) A.aw_out[(size_t)v * K + a] = acc[a];
+        return;
+    }
+    if (bottom) {
+        TA e = TA(0);
+#pragma unroll
+        for (int l = 0; l <= L; ++l) e += A.coefE[l] * acc[l * l + l];
+        S.dE[pix] = e;
+    }
+    if (S.mode == 0) {
+        const TA ms = TA(S.mus[v]);
+        TA gl = TA(1), bs = TA(0);
+#pragma unroll
+        for (int l = 0; l <= L; ++l) {
+#pragma unroll
+            for (int m = -l; m <= l; ++m) {
+                const int a = l * l + l + m;
+                const TA wn = gl * ms * acc[a];
+                const T wt = T(wn * A.nsc[a]);
+                S.W[(size_t)v * K + a] = wt;
+                bs += A.wsum[a] * (TA(wt) / A.nsc[a]);
+            }
+            gl *= TA(S.g);
+        }
+        if (bottom) S.bsum[pix] = bs;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-solve reduction of the bottom face in a fixed order (deterministic,
+// independent of the batch).  Per-solve mode:
+//   0 (term):  E += dE; red = (sum dE, sum bsum)
+//   1 (init):  red = (sum E, sum bsum)          (after the source)
+//   2 (final): E += dE; red = (sum dE, 0)
+// grid = nsolve, block = 256.
+// ---------------------------------------------------------------------------
+template <typename T, typename TA>
+__global__ __launch_bounds__(256) void k_bottom(int P, const SolveDev<T, TA> *__restrict__ sv)
+{
+    __shared__ TA s1[256], s2[256];
+    const SolveDev<T, TA> S = sv[blockIdx.x];
+    TA a = TA(0), b = TA(0);
+    for (int p = threadIdx.x; p < P; p += 256) {
+        if (S.mode == 1) {
+            a += S.E[p];
+        } else {
+            const TA d = S.dE[p];
+            S.E[p] += d;
+            a += d;
+        }
+        if (S.mode != 2) b += S.bsum[p];
+    }
+    s1[threadIdx.x] = a;
+    s2[threadIdx.x] = b;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            s1[threadIdx.x] += s1[threadIdx.x + h];
+            s2[threadIdx.x] += s2[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        S.red[0] = s1[0];
+        S.red[1] = s2[0];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// First-scatter source s0 and ballistic image (column products, FP64):
+//   s0_l0(i,j,k) = fact g^l sqrt((2l+1)/4pi) (mu_s/mu_t) T(k) (1 - e^{-mu_t dz}) / dV
+// (GPU package/src/dist.cu:66-101; CPU package/src/Dist.c:1077-1128),
+// ballistic = fact T(nz) (GPU package/src/dist.cu:129-141).  W must be zero.
+// grid = (ceil(P/64), nsolve), block = 64.
+// ---------------------------------------------------------------------------
+template <typename T, typename TA, int L>
+__global__ __launch_bounds__(kWave) void k_source(GridDev<T> g, double dxd, double dyd, double dzd,
+                                                 double xmin, double ymin, Scales sc,
+                                                 const BeamDev *__restrict__ beams,
+                                                 const SolveDev<T, TA> *__restrict__ sv)
+{
+    constexpr int K = (L + 1) * (L + 1);
+    const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int P = g.nx * g.ny;
+    if (pix >= P) return;
+    const BeamDev B = beams[blockIdx.y];
+    const SolveDev<T, TA> S = sv[blockIdx.y];
+    const int i = pix % g.nx, j = pix / g.nx;
+    const double xc = (double)i * dxd + xmin + dxd / 2.0;
+    const double yc = (double)j * dyd + ymin + dyd / 2.0;
+    const double rad = sqrt((xc - B.pos_x) * (xc - B.pos_x) + (yc - B.pos_y) * (yc - B.pos_y));
+    const bool in = rad <= B.half_diam;
+    const double dV = dxd * dyd * dzd;
+    const double gg = (double)S.g;
+    double Tc = 1.0;
+    double bs = 0.0;
+    if (in) {
+        for (int k = 0; k < g.nz; ++k) {
+            const int v = pix + P * k;
+            const double mt = B.mua[v] + B.mus[v];
+            const double att = exp(-mt * dzd);
+            const double base = B.fact * (B.mus[v] / mt) * Tc * (1.0 - att) / dV;
+            double gl = 1.0;
+#pragma unroll
+            for (int l = 0; l <= L; ++l) {
+                const double s = gl * sqrt((2.0 * l + 1.0) / (4.0 * M_PI)) * base;
+                const T st = T(s * sc.n[l * l + l]);
+                S.W[(size_t)v * K + l * l + l] = st;
+                if (k == g.nz - 1) bs += (double)st / sc.n[l * l + l];
+                gl *= gg;
+            }
+            Tc *= att;
+        }
+    }
+    S.E[pix] = TA(in ? B.fact * Tc : 0.0);
+    S.bsum[pix] = TA(bs);
+    S.dE[pix] = TA(0);
+}
+
+// ---------------------------------------------------------------------------
+// Self-block moments M_k[a][b] = sum_w R_a(w) R_b(w) sin(theta_w) beta_w^k,
+// k = 1..kNMom, on the reference's 200x200 grid.  grid = K*K, block = 256.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_moments(int K, const double *__restrict__ R /*[NW][K]*/,
+                                                 const double *__restrict__ sn,
+                                                 const double *__restrict__ beta,
+                                                 double *__restrict__ M)
+{
+    __shared__ double red[kNMom][256];
+    const int a = blockIdx.x / K, b = blockIdx.x % K;
+    double acc[kNMom];
+#pragma unroll
+    for (int k = 0; k < kNMom; ++k) acc[k] = 0.0;
+    for (int w = threadIdx.x; w < kNOmega; w += 256) {
+        const double f = R[(size_t)w * K + a] * R[(size_t)w * K + b] * sn[w];
+        double bk = beta[w];
+#pragma unroll
+        for (int k = 0; k < kNMom; ++k) {
+            acc[k] += f * bk;
+            bk *= beta[w];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kNMom; ++k) red[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+#pragma unroll
+            for (int k = 0; k < kNMom; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < kNMom) M[((size_t)threadIdx.x * K + a) * K + b] = red[threadIdx.x][0];
+}
+
+// ---------------------------------------------------------------------------
+// Layout conversions for the operator entry points and small operators
+// ---------------------------------------------------------------------------
+// coefficient-major double [K][N]  ->  voxel-major T [N][K]
+template <typename T>
+__global__ void k_to_voxel_major(int K, int N, Scales sc, const double *__restrict__ in, T *__restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)K * N) return;
+    const int v = (int)(i / K), a = (int)(i % K);
+    out[i] = T(in[(size_t)a * N + v] * sc.n[a]);
+}
+
+template <typename TA>
+__global__ void k_to_coef_major(int K, int N, Scales sc, const TA *__restrict__ in, double *__restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)K * N) return;
+    const int a = (int)(i / N), v = (int)(i % N);
+    out[i] = (double)in[(size_t)v * K + a] / sc.n[a];
+}
+
+// D on a coefficient-major double field: W_a(v) *= g^l mu_s(v)
+__global__ void k_scatter_op(int L, int N, double g, const double *__restrict__ mus,
+                             double *__restrict__ W)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int K = (L + 1) * (L + 1);
+    if (i >= (size_t)K * N) return;
+    const int a = (int)(i / N), v = (int)(i % N);
+    int l = 0;
+    while ((l + 1) * (l + 1) <= a) ++l;
+    W[i] *= pow(g, (double)l) * mus[v];
+}
+
+// exitance of a coefficient-major double field (ballistic excluded)
+__global__ void k_exitance_op(int L, int nx, int ny, int nz, const double *__restrict__ coefE,
+                              const double *__restrict__ w, double *__restrict__ img)
+{
+    const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int P = nx * ny, N = P * nz;
+    if (pix >= P) return;
+    const int v = pix + P * (nz - 1);
+    double e = 0.0;
+    for (int l = 0; l <= L; ++l) e += coefE[l] * w[(size_t)(l * l + l) * N + v];
+    img[pix] = e;
+}
+
+// tau for a list of pairs, marched in FP32 as the kernels do
+__global__ void k_tau_list(GridDev<float> g, const float *__restrict__ mut, int np,
+                           const int *__restrict__ dst, const int *__restrict__ src,
+                           double *__restrict__ tau)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const int d = dst[i], s = src[i];
+    const int nxy = g.nx * g.ny;
+    const int di = d % g.nx, dj = (d / g.nx) % g.ny, dk = d / nxy;
+    const int si = s % g.nx, sj = (s / g.nx) % g.ny, sk = s / nxy;
+    const int ix = di - si, iy = dj - sj, iz = dk - sk;
+    const float ddx = float(ix) * g.dx, ddy = float(iy) * g.dy, ddz = float(iz) * g.dz;
+    const float r = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);
+    tau[i] = (d == s) ? 0.0 : (double)march_tau<float>(as_global<float>(mut), s, ix, iy, iz, g.nx, nxy, r);
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers (dispatch over L)
+// ---------------------------------------------------------------------------
+#define RTE_DISPATCH_L(L, ...)                          \
+    switch (L) {                                        \
+    case 0: { constexpr int LL = 0; __VA_ARGS__; } break; \
+    case 1: { constexpr int LL = 1; __VA_ARGS__; } break; \
+    case 2: { constexpr int LL = 2; __VA_ARGS__; } break; \
+    case 3: { constexpr int LL = 3; __VA_ARGS__; } break; \
+    case 4: { constexpr int LL = 4; __VA_ARGS__; } break; \
+    case 5: { constexpr int LL = 5; __VA_ARGS__; } break; \
+    case 6: { constexpr int LL = 6; __VA_ARGS__; } break; \
+    case 7: { constexpr int LL = 7; __VA_ARGS__; } break; \
+    default: return hipErrorInvalidValue;                \
+    }
+
+template <typename T>
+hipError_t launch_fill(T *p, size_t n, T v, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+template <typename T, typename TA>
+hipError_t launch_prop(int L, int path, const PropArgs<T> &A, const SolveDev<T, TA> *sv, int nsolve,
+                       hipStream_t st)
+{
+    dim3 grid(A.g.nblk, A.nchunk, nsolve), block(kWave);
+    RTE_DISPATCH_L(L, {
+        if (path == 0)
+            hipLaunchKernelGGL((k_prop<T, TA, LL, 0>), grid, block, 0, st, A, sv);
+        else
+            hipLaunchKernelGGL((k_prop<T, TA, LL, 1>), grid, block, 0, st, A, sv);
+    });
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_ktab(const GridDev<T> &g, int nchunk, int chunk_len, T thr2, const T *mut, T *ktab,
+                       hipStream_t st)
+{
+    hipLaunchKernelGGL(k_ktab<T>, dim3(g.nblk, nchunk), dim3(kWave), 0, st, g, chunk_len, thr2, mut,
+                       ktab);
+    return hipGetLastError();
+}
+
+template <typename T, typename TA>
+hipError_t launch_combine(int L, const CombArgs<T, TA> &A, const SolveDev<T, TA> *sv, int nsolve,
+                          hipStream_t st)
+{
+    dim3 grid((A.g.N + kWave - 1) / kWave, nsolve), block(kWave);
+    RTE_DISPATCH_L(L, hipLaunchKernelGGL((k_combine<T, TA, LL>), grid, block, 0, st, A, sv));
+    return hipGetLastError();
+}
+
+template <typename T, typename TA>
+hipError_t launch_bottom(int P, const SolveDev<T, TA> *sv, int nsolve, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_bottom<T, TA>), dim3(nsolve), dim3(256), 0, st, P, sv);
+    return hipGetLastError();
+}
+
+template <typename T, typename TA>
+hipError_t launch_source(int L, const GridDev<T> &g, double dx, double dy, double dz, double xmin,
+                         double ymin, const Scales &sc, const BeamDev *beams, const SolveDev<T, TA> *sv,
+                         int nsolve, hipStream_t st)
+{
+    dim3 grid((g.nx * g.ny + kWave - 1) / kWave, nsolve), block(kWave);
+    RTE_DISPATCH_L(L, hipLaunchKernelGGL((k_source<T, TA, LL>), grid, block, 0, st, g, dx, dy, dz,
+                                         xmin, ymin, sc, beams, sv));
+    return hipGetLastError();
+}
+
+hipError_t launch_moments(int K, const double *R, const double *sn, const double *beta, double *M,
+                          hipStream_t st)
+{
+    hipLaunchKernelGGL(k_moments, dim3(K * K), dim3(256), 0, st, K, R, sn, beta, M);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_to_voxel_major(int K, int N, const Scales &sc, const double *in, T *out, hipStream_t st)
+{
+    size_t n = (size_t)K * N;
+    hipLaunchKernelGGL(k_to_voxel_major<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, K, N,
+                       sc, in, out);
+    return hipGetLastError();
+}
+
+template <typename TA>
+hipError_t launch_to_coef_major(int K, int N, const Scales &sc, const TA *in, double *out, hipStream_t st)
+{
+    size_t n = (size_t)K * N;
+    hipLaunchKernelGGL(k_to_coef_major<TA>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, K, N,
+                       sc, in, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_op(int L, int N, double g, const double *mus, double *W, hipStream_t st)
+{
+    size_t n = (size_t)(L + 1) * (L + 1) * N;
+    hipLaunchKernelGGL(k_scatter_op, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, L, N, g, mus,
+                       W);
+    return hipGetLastError();
+}
+
+hipError_t launch_exitance_op(int L, int nx, int ny, int nz, const double *coefE, const double *w,
+                              double *img, hipStream_t st)
+{
+    int P = nx * ny;
+    hipLaunchKernelGGL(k_exitance_op, dim3((P + 255) / 256), dim3(256), 0, st, L, nx, ny, nz, coefE, w,
+                       img);
+    return hipGetLastError();
+}
+
+hipError_t launch_tau_list(const GridDev<float> &g, const float *mut, int np, const int *dst,
+                           const int *src, double *tau, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_tau_list, dim3((np + 255) / 256), dim3(256), 0, st, g, mut, np, dst, src, tau);
+    return hipGetLastError();
+}
+
+// explicit instantiations used by the host library
+template hipError_t launch_fill<float>(float *, size_t, float, hipStream_t);
+template hipError_t launch_fill<double>(double *, size_t, double, hipStream_t);
+template hipError_t launch_prop<float, double>(int, int, const PropArgs<float> &,
+                                               const SolveDev<float, double> *, int, hipStream_t);
+template hipError_t launch_prop<double, double>(int, int, const PropArgs<double> &,
+                                                const SolveDev<double, double> *, int, hipStream_t);
+template hipError_t launch_prop<float, float>(int, int, const PropArgs<float> &,
+                                              const SolveDev<float, float> *, int, hipStream_t);
+template hipError_t launch_ktab<float>(const GridDev<float> &, int, int, float, const float *, float *,
+                                       hipStream_t);
+template hipError_t launch_ktab<double>(const GridDev<double> &, int, int, double, const double *,
+                                        double *, hipStream_t);
+template hipError_t launch_combine<float, double>(int, const CombArgs<float, double> &,
+                                                  const SolveDev<float, double> *, int, hipStream_t);
+template hipError_t launch_combine<double, double>(int, const CombArgs<double, double> &,
+                                                   const SolveDev<double, double> *, int, hipStream_t);
+template hipError_t launch_combine<float, float>(int, const CombArgs<float, float> &,
+                                                 const SolveDev<float, float> *, int, hipStream_t);
+template hipError_t launch_bottom<float, double>(int, const SolveDev<float, double> *, int,
+                                                 hipStream_t);
+template hipError_t launch_bottom<double, double>(int, const SolveDev<double, double> *, int,
+                                                  hipStream_t);
+template hipError_t launch_bottom<float, float>(int, const SolveDev<float, float> *, int,
+                                                hipStream_t);
+template hipError_t launch_source<float, double>(int, const GridDev<float> &, double, double, double,
+                                                 double, double, const Scales &, const BeamDev *,
+                                                 const SolveDev<float, double> *, int, hipStream_t);
+template hipError_t launch_source<double, double>(int, const GridDev<double> &, double, double, double,
+                                                  double, double, const Scales &, const BeamDev *,
+                                                  const SolveDev<double, double> *, int, hipStream_t);
+template hipError_t launch_source<float, float>(int, const GridDev<float> &, double, double, double,
+                                                double, double, const Scales &, const BeamDev *,
+                                                const SolveDev<float, float> *, int, hipStream_t);
+template hipError_t launch_to_voxel_major<float>(int, int, const Scales &, const double *, float *, hipStream_t);
+template hipError_t launch_to_voxel_major<double>(int, int, const Scales &, const double *, double *,
+                                                  hipStream_t);
+template hipError_t launch_to_coef_major<float>(int, int, const Scales &, const float *, double *, hipStream_t);
+template hipError_t launch_to_coef_major<double>(int, int, const Scales &, const double *, double *,
+                                                 hipStream_t);
+
+}  // namespace rte

@@ -1,0 +1,246 @@
+"""ctypes binding of librte.so (include/rte.h).
+
+Mirrors the reference solver's entry points: a grid, optical maps, a pencil
+beam and the SH order in; the bottom-face exitance out
+(CPU package/src/RTE_uniform.c:79-181).  Field arrays at this boundary are
+coefficient-major ``W[a, v]`` in the real SH basis, ``v = x + nx*(y + ny*z)``.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librte.so")
+
+OK, E_ARG, E_DEVICE, E_NOMEM, E_UNSUPP, E_BUSY, E_NOTFOUND = 0, -1, -2, -3, -4, -5, -6
+MAX_L = 7
+STOP_LAMBDA, STOP_DELTA = 0, 1
+PREC_FP64, PREC_MIXED, PREC_FP32 = 0, 1, 2
+QUAD_REF, QUAD_EXACT = 0, 1
+PATH_CACHED, PATH_MARCH = 0, 1
+
+
+class RteError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("librte error %d: %s" % (code, msg))
+        self.code = code
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("xmin", C.c_double), ("xmax", C.c_double),
+                ("ymin", C.c_double), ("ymax", C.c_double),
+                ("zmin", C.c_double), ("zmax", C.c_double)]
+
+
+class Optics(C.Structure):
+    _fields_ = [("mu_a", C.POINTER(C.c_double)), ("mu_s", C.POINTER(C.c_double)),
+                ("g", C.c_double), ("n", C.c_double), ("c", C.c_double)]
+
+
+class Beam(C.Structure):
+    _fields_ = [("pos_x", C.c_double), ("pos_y", C.c_double),
+                ("diam", C.c_double), ("mag", C.c_double)]
+
+
+class Params(C.Structure):
+    _fields_ = [("L", C.c_int), ("max_terms", C.c_int), ("stop_rule", C.c_int),
+                ("tol", C.c_double), ("prop_thresh", C.c_double), ("precision", C.c_int),
+                ("exit_quad", C.c_int), ("path_mode", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("terms", C.c_int), ("lambda_last", C.c_double), ("rel_change_last", C.c_double),
+                ("total_exitance", C.c_double), ("ballistic_total", C.c_double),
+                ("gpu_ms", C.c_double)]
+
+
+# every entry point of include/rte.h: name -> (restype, argtypes)
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_vp = C.c_void_p
+SIGNATURES = {
+    "rte_version": (C.c_int, []),
+    "rte_last_error": (C.c_char_p, []),
+    "rte_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "rte_ctx_destroy": (C.c_int, [_vp]),
+    "rte_medium_create": (C.c_int, [_vp, C.POINTER(Grid), C.POINTER(Optics), C.POINTER(_vp)]),
+    "rte_medium_destroy": (C.c_int, [_vp]),
+    "rte_medium_reset_cache": (C.c_int, [_vp]),
+    "rte_solve_footprint": (C.c_longlong, [C.POINTER(Grid), C.POINTER(Params), C.c_int]),
+    "rte_solve": (C.c_int, [_vp, C.POINTER(Grid), C.POINTER(Optics), C.POINTER(Beam),
+                            C.POINTER(Params), _dp, C.POINTER(Stats)]),
+    "rte_solve_media": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(Beam), C.POINTER(Params),
+                                  _dp, C.POINTER(Stats)]),
+    "rte_propagate": (C.c_int, [_vp, C.POINTER(Grid), C.POINTER(Optics), C.c_int, C.c_double,
+                                C.c_int, C.c_int, _dp, _dp]),
+    "rte_scatter": (C.c_int, [_vp, C.POINTER(Grid), C.POINTER(Optics), C.c_int, _dp]),
+    "rte_source": (C.c_int, [_vp, C.POINTER(Grid), C.POINTER(Optics), C.POINTER(Beam), C.c_int,
+                             _dp, _dp, _ip]),
+    "rte_exitance": (C.c_int, [_vp, C.POINTER(Grid), C.c_int, C.c_int, C.c_double, _dp, _dp]),
+    "rte_path_integrals": (C.c_int, [_vp, C.POINTER(Grid), C.POINTER(Optics), C.c_int, _ip, _ip, _dp]),
+}
+
+_LIB = None
+
+
+def lib():
+    """Loads librte.so; raises if it is missing (no fallback path exists)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("librte.so not built: run `python -m doi_amd.build` (%s)" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc != OK:
+        raise RteError(rc, lib().rte_last_error().decode(errors="replace"))
+    return rc
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def make_params(L, precision=PREC_MIXED, stop_rule=STOP_LAMBDA, tol=1e-6, max_terms=1000,
+                prop_thresh=0.0, exit_quad=QUAD_REF, path_mode=PATH_CACHED):
+    return Params(L, max_terms, stop_rule, tol, prop_thresh, precision, exit_quad, path_mode)
+
+
+class Medium:
+    """Optical maps of one grid, resident on the device."""
+
+    def __init__(self, ctx, shape, bounds, mu_a, mu_s, g, n=1.0, c=2.9979e10):
+        nx, ny, nz = shape
+        self.ctx = ctx
+        self.shape = (nx, ny, nz)
+        self.N = nx * ny * nz
+        self.grid = Grid(nx, ny, nz, *bounds)
+        self.mu_a = _f64(np.broadcast_to(mu_a, (self.N,)) if np.ndim(mu_a) == 0 else np.ravel(mu_a))
+        self.mu_s = _f64(np.broadcast_to(mu_s, (self.N,)) if np.ndim(mu_s) == 0 else np.ravel(mu_s))
+        self.optics = Optics(_p(self.mu_a), _p(self.mu_s), g, n, c)
+        self.h = _vp()
+        _check(lib().rte_medium_create(ctx.h, C.byref(self.grid), C.byref(self.optics), C.byref(self.h)))
+
+    def reset_cache(self):
+        _check(lib().rte_medium_reset_cache(self.h))
+
+    def close(self):
+        if self.h:
+            lib().rte_medium_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context:
+    """One device; owns a HIP stream and cached per-shape tables."""
+
+    def __init__(self, device=0):
+        self.h = _vp()
+        _check(lib().rte_ctx_create(device, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().rte_ctx_destroy(self.h)
+            self.h = _vp()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def medium(self, shape, bounds, mu_a, mu_s, g, n=1.0, c=2.9979e10):
+        return Medium(self, shape, bounds, mu_a, mu_s, g, n, c)
+
+    def solve_media(self, media, beams, params):
+        nb = len(media)
+        nx, ny, _ = media[0].shape
+        arr = (_vp * nb)(*[m.h for m in media])
+        barr = (Beam * nb)(*[Beam(*b) for b in beams])
+        img = np.zeros(nb * nx * ny)
+        st = (Stats * nb)()
+        _check(lib().rte_solve_media(self.h, nb, arr, barr, C.byref(params), _p(img), st))
+        return img.reshape(nb, ny, nx), [dict(terms=s.terms, lam=s.lambda_last, rel=s.rel_change_last,
+                                              total=s.total_exitance, ballistic=s.ballistic_total,
+                                              gpu_ms=s.gpu_ms) for s in st]
+
+    def solve(self, shape, bounds, mu_a, mu_s, g, beam, params, n=1.0, c=2.9979e10):
+        m = self.medium(shape, bounds, mu_a, mu_s, g, n, c)
+        try:
+            img, st = self.solve_media([m], [beam], params)
+        finally:
+            m.close()
+        return img[0], st[0]
+
+    # ---- operators -------------------------------------------------------
+    def propagate(self, shape, bounds, mu_a, mu_s, g, W, L, prop_thresh=0.0, precision=PREC_FP64,
+                  path_mode=PATH_CACHED, n=1.0, c=2.9979e10):
+        m = _Maps(shape, bounds, mu_a, mu_s, g, n, c)
+        W = _f64(W)
+        out = np.zeros_like(W)
+        _check(lib().rte_propagate(self.h, C.byref(m.grid), C.byref(m.optics), L, prop_thresh,
+                                   precision, path_mode, _p(W), _p(out)))
+        return out
+
+    def scatter(self, shape, bounds, mu_a, mu_s, g, W, L, n=1.0, c=2.9979e10):
+        m = _Maps(shape, bounds, mu_a, mu_s, g, n, c)
+        W = _f64(W).copy()
+        _check(lib().rte_scatter(self.h, C.byref(m.grid), C.byref(m.optics), L, _p(W)))
+        return W
+
+    def source(self, shape, bounds, mu_a, mu_s, g, beam, L, n=1.0, c=2.9979e10):
+        m = _Maps(shape, bounds, mu_a, mu_s, g, n, c)
+        K = (L + 1) ** 2
+        s0 = np.zeros((K, m.N))
+        ball = np.zeros(shape[0] * shape[1])
+        ncols = C.c_int(0)
+        _check(lib().rte_source(self.h, C.byref(m.grid), C.byref(m.optics), C.byref(Beam(*beam)), L,
+                                _p(s0), _p(ball), C.byref(ncols)))
+        return s0, ball, ncols.value
+
+    def exitance(self, shape, bounds, w, L, quad=QUAD_REF, c_m=2.9979e10):
+        grid = Grid(*shape, *bounds)
+        img = np.zeros(shape[0] * shape[1])
+        _check(lib().rte_exitance(self.h, C.byref(grid), L, quad, c_m, _p(_f64(w)), _p(img)))
+        return img
+
+    def path_integrals(self, shape, bounds, mu_a, mu_s, dest, src):
+        m = _Maps(shape, bounds, mu_a, mu_s, 0.0, 1.0, 1.0)
+        d = np.ascontiguousarray(dest, dtype=np.int32)
+        s = np.ascontiguousarray(src, dtype=np.int32)
+        tau = np.zeros(d.size)
+        _check(lib().rte_path_integrals(self.h, C.byref(m.grid), C.byref(m.optics), d.size,
+                                        d.ctypes.data_as(_ip), s.ctypes.data_as(_ip), _p(tau)))
+        return tau
+
+
+class _Maps:
+    def __init__(self, shape, bounds, mu_a, mu_s, g, n, c):
+        nx, ny, nz = shape
+        self.N = nx * ny * nz
+        self.grid = Grid(nx, ny, nz, *bounds)
+        self.mu_a = _f64(np.broadcast_to(mu_a, (self.N,)) if np.ndim(mu_a) == 0 else np.ravel(mu_a))
+        self.mu_s = _f64(np.broadcast_to(mu_s, (self.N,)) if np.ndim(mu_s) == 0 else np.ravel(mu_s))
+        self.optics = Optics(_p(self.mu_a), _p(self.mu_s), g, n, c)
+
+
+def footprint(shape, bounds, params, path_mode=PATH_CACHED):
+    return lib().rte_solve_footprint(C.byref(Grid(*shape, *bounds)), C.byref(params), path_mode)

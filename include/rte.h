@@ -1,0 +1,167 @@
+/*
+ * rte.h -- C ABI of the MI355X Neumann-series RTE solver (librte.so).
+ *
+ * Drop-in boundary for the reference's solver seam: the reference has no
+ * plugin API; its solve is the C function
+ *     void Neumann(complex_double *src, complex_double *out, int flag)
+ * (FisherInfo_IO/src/rte.h:187, FisherInfo_IO/src/neumann.cu:23; GPU package/
+ * src/neumann.cu:23) over process globals set up by each main
+ * (GPU package/src/rte.cu:11-18), plus the CPU package's operator functions
+ * PropAbs2 / PropScat / GenerateSourceBeam / ComputeTransImage / StopIteration
+ * (CPU package/src/RTE.h:98-128).  This header replaces them with re-entrant
+ * entry points over POD structs.  Every function returns RTE_OK (0) or a
+ * negative RTE_E* code and never exits the process (the reference calls
+ * exit() on any failure, GPU package/src/defines.h:24-44); rte_last_error()
+ * gives the message of the last failure on the calling thread.
+ *
+ * Units: cm and cm^-1.  Exitance is photons per incident photon on the bottom
+ * face (GPU package/GPUManual.pdf p.12).
+ *
+ * Conventions:
+ *   voxel v = x + nx*(y + ny*z), voxel centre x_i = xmin + (i + 1/2) dx,
+ *     dx = (xmax - xmin)/nx                  (CPU package/src/Inits.c:106-122)
+ *   fields (rte_propagate & co.) are coefficient-major W[a*N + v], N = nx*ny*nz,
+ *     a = l*l + l + m, in the REAL orthonormal SH basis
+ *     R_l0 = Y_l0, R_lm = sqrt2 Re Y_lm, R_l,-m = sqrt2 Im Y_lm (m > 0);
+ *     for a real radiance this is a unitary change of the reference's complex
+ *     coefficients (CPU package/src/Dist.c:1207-1216) that leaves every m = 0
+ *     coefficient, and so the exitance, unchanged
+ *   exitance images are img[x + nx*y]  (CPU package/src/Dist.c:24-36 order)
+ *
+ * Threading: a context is used by one host thread at a time.  Device memory
+ * belongs to the context or medium that allocated it; host buffers to the
+ * caller.
+ */
+#ifndef RTE_H
+#define RTE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RTE_OK 0
+#define RTE_E_ARG -1      /* invalid argument (sizes, NaN/negative optics, ...) */
+#define RTE_E_DEVICE -2   /* HIP runtime error */
+#define RTE_E_NOMEM -3    /* device or host allocation failed */
+#define RTE_E_UNSUPP -4   /* combination not supported (e.g. L > RTE_MAX_L) */
+#define RTE_E_BUSY -5     /* server queue full (back-pressure) */
+#define RTE_E_NOTFOUND -6 /* unknown request id */
+
+#define RTE_MAX_L 7
+
+/* Geometry (CPU package/src/RTE.h:44-75; CPU package/src/Inits.c:76-134). */
+typedef struct {
+    int nx, ny, nz;
+    double xmin, xmax, ymin, ymax, zmin, zmax;
+} rte_grid;
+
+/* Optical maps (reference: uniform Phantom, CPU package/src/RTE.h:86-92, or a
+ * tissue byte map with <= 3 mu pairs, GPU package/src/rte.h:54-60; here one
+ * value per voxel).  c_m = c/n cancels from the exitance and is not used in
+ * arithmetic; n and c are accepted for interface fidelity and validated. */
+typedef struct {
+    const double *mu_a; /* N values, >= 0 */
+    const double *mu_s; /* N values, >= 0, mu_a + mu_s > 0 */
+    double g;           /* Henyey-Greenstein anisotropy, |g| <= 1 */
+    double n;           /* refractive index, > 0 */
+    double c;           /* speed of light, > 0 */
+} rte_optics;
+
+/* Collimated pencil beam on the top face (CPU package/src/RTE.h:77-86).  The
+ * columns with |centre - pos| <= (diam + 1e-16)/2 carry mag/#columns each
+ * (CPU package/src/Inits.c:56, Dist.c:1090-1114). */
+typedef struct {
+    double pos_x, pos_y, diam, mag;
+} rte_beam;
+
+#define RTE_STOP_LAMBDA 0 /* reference rule: lambda > 0.9999 (CPU package/src/Kernels_selfsub.c:1184-1233) */
+#define RTE_STOP_DELTA 1  /* |change of total exitance| < tol * |total exitance| */
+
+#define RTE_PREC_FP64 0  /* everything in FP64 */
+#define RTE_PREC_MIXED 1 /* propagation/scatter FP32; running sums, exitance FP64 */
+#define RTE_PREC_FP32 2  /* everything FP32 */
+
+#define RTE_QUAD_REF 0   /* reference 100-point left Riemann sum (CPU package/src/Dist.c:190-218) */
+#define RTE_QUAD_EXACT 1 /* exact int_0^1 P_l(u) u du */
+
+#define RTE_PATH_CACHED 0 /* path integrals computed once per medium, reused by every term */
+#define RTE_PATH_MARCH 1  /* path integral marched per pair in every term (reference structure) */
+
+typedef struct {
+    int L;              /* SH order, 0 <= L <= RTE_MAX_L */
+    int max_terms;      /* cap on Neumann terms; <= 0 means 1000 */
+    int stop_rule;      /* RTE_STOP_* */
+    double tol;         /* for RTE_STOP_DELTA */
+    double prop_thresh; /* pairs with r >= prop_thresh are not coupled; <= 0 = no limit */
+    int precision;      /* RTE_PREC_* */
+    int exit_quad;      /* RTE_QUAD_* */
+    int path_mode;      /* RTE_PATH_* */
+} rte_params;
+
+typedef struct {
+    int terms;              /* the reference's "Neumann series: n" */
+    double lambda_last;     /* last lambda of the reference stop rule */
+    double rel_change_last; /* last relative change of total exitance */
+    double total_exitance;  /* sum of the image incl. ballistic */
+    double ballistic_total; /* sum of the ballistic image */
+    double gpu_ms;          /* device time of the solve (events on the solve stream) */
+} rte_stats;
+
+typedef struct rte_ctx rte_ctx;
+typedef struct rte_medium rte_medium;
+
+/* ---- context ------------------------------------------------------------ */
+int rte_ctx_create(int device, rte_ctx **out);
+int rte_ctx_destroy(rte_ctx *ctx);
+const char *rte_last_error(void);
+int rte_version(void);
+
+/* ---- device-resident media ---------------------------------------------
+ * Uploads the optical maps once; solves on a medium reuse its device copy and
+ * its per-medium tables (path integrals, self blocks) across calls. */
+int rte_medium_create(rte_ctx *ctx, const rte_grid *grid, const rte_optics *optics,
+                      rte_medium **out);
+int rte_medium_destroy(rte_medium *m);
+/* drops the per-medium tables (the next solve rebuilds them) */
+int rte_medium_reset_cache(rte_medium *m);
+/* bytes of device memory a solve of this shape needs (admission by HBM footprint) */
+long long rte_solve_footprint(const rte_grid *grid, const rte_params *params, int path_mode);
+
+/* ---- solves --------------------------------------------------------------
+ * Source -> Neumann series -> bottom-face exitance, replacing the reference
+ * main's GenerateSourceBeam / Neumann / ComputeTransImage sequence
+ * (CPU package/src/RTE_uniform.c:79-181; GPU package/src/rte.cu:136-203). */
+int rte_solve(rte_ctx *ctx, const rte_grid *grid, const rte_optics *optics,
+              const rte_beam *beam, const rte_params *params, double *exitance /* nx*ny */,
+              rte_stats *stats);
+
+/* nb solves on device-resident media of one shape (same grid dims and L);
+ * media[i] may repeat (several beams on one medium share its tables and their
+ * propagations).  exitance is nb*nx*ny, stats nb entries (may be NULL). */
+int rte_solve_media(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_beam *beams,
+                    const rte_params *params, double *exitance, rte_stats *stats);
+
+/* ---- operators (for operator-level parity) ------------------------------ */
+/* A: propagation incl. self-voxel block and dV (CPU package/src/Kernels_selfsub.c:450-538,
+ * GPU package/src/kernels.cu:148-175,286-381).  W, out: K*N doubles. */
+int rte_propagate(rte_ctx *ctx, const rte_grid *grid, const rte_optics *optics, int L,
+                  double prop_thresh, int precision, int path_mode, const double *W,
+                  double *out);
+/* D: Henyey-Greenstein scatter g^l mu_s (CPU package/src/Kernels_selfsub.c:30-51), in place */
+int rte_scatter(rte_ctx *ctx, const rte_grid *grid, const rte_optics *optics, int L, double *W);
+/* s0 (K*N) and the ballistic image (nx*ny) (CPU package/src/Dist.c:1077-1128, 203-209;
+ * GPU package/src/dist.cu:66-101, 129-141); returns #beam columns in *ncols */
+int rte_source(rte_ctx *ctx, const rte_grid *grid, const rte_optics *optics, const rte_beam *beam,
+               int L, double *s0, double *ballistic, int *ncols);
+/* bottom-face exitance of a field w (K*N), without the ballistic term; c_m =
+ * c/n is the speed of light in the medium (CPU package/src/Dist.c:178-229) */
+int rte_exitance(rte_ctx *ctx, const rte_grid *grid, int L, int exit_quad, double c_m,
+                 const double *w, double *img);
+/* path integral tau(d, s) for a list of voxel pairs (GPU package/src/kernels.cu:49-145) */
+int rte_path_integrals(rte_ctx *ctx, const rte_grid *grid, const rte_optics *optics, int npairs,
+                       const int *dest, const int *src, double *tau);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
