@@ -1,0 +1,182 @@
+"""Benchmark: RTE solves/s and p50/p99 latency at 32^3, L=5 (mixed FP32/FP64),
+with the exitance error against an all-FP64 solve of the same inputs.
+
+Workload (one "step"): 32 independent heterogeneous media of 32^3 voxels
+(0.1 cm), L=5, g=0.9, mu_s=1, mu_a=0.01, lognormal sigma=0.3, 1-3 spherical
+mu_a inclusions, 5 pencil beams per medium -> 160 solves, Neumann series to a
+1e-6 relative change of total exitance (cap 30 terms).  The media are resident
+in HBM before the timed region; every step rebuilds their path tables.
+
+    python bench.py [--gpus 1] [--steps 2] [--warmup 1]
+
+Prints one JSON line.  One GPU only: --gpus N > 1 exits with a message.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 (vector/matrix dense) published peak
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(shape, bounds, mu_a, mu_s, g, L, terms, budget_s=12.0):
+    """FP64 oracle (C, OpenMP) timed on this host on a sample of destination
+    voxels of one propagation; scaled to solves/s with the GPU's term count."""
+    from oracle import oracle as O
+    N = int(np.prod(shape))
+    P = O.Problem(shape, bounds, mu_a, mu_s, g)
+    W = np.random.default_rng(0).standard_normal(((L + 1) ** 2, N))
+    t0 = time.perf_counter()
+    P.propagate_range(W, L, 0, 0)          # table set-up only
+    setup = time.perf_counter() - t0
+    rows, done, el = 256, 0, 0.0
+    while el < budget_s and done < N:
+        n = min(rows, N - done)
+        t0 = time.perf_counter()
+        P.propagate_range(W, L, done, done + n)
+        el += time.perf_counter() - t0 - setup
+        done += n
+        rows *= 2
+    per_prop = el * N / done
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=1.0 / (per_prop * terms), unit="solves/s", cores=cores, kind="port",
+                sample="FP64 oracle propagation of one 32^3 L=5 medium over %d of %d destination voxels "
+                       "(%.1f s), scaled to a full propagation (%.1f s) x %.1f Neumann terms per solve"
+                       % (done, N, el, per_prop, terms))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--media", type=int, default=32)
+    ap.add_argument("--beams", type=int, default=5)
+    ap.add_argument("--n", type=int, default=32)
+    ap.add_argument("--L", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (independent solves "
+              "shard across GPUs with no communication; see DESIGN.md). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from doi_amd import media, rte
+
+    n, L = args.n, args.L
+    shape, bounds = (n, n, n), media.cube_bounds(n)
+    g = 0.9
+    log("generating %d media of %d^3" % (args.media, n))
+    maps = media.config5_media(args.media, seed=5, n=n)
+    beams5 = media.config5_beams(seed=5, n=n, per_medium=args.beams)
+    ctx = rte.Context(0)
+    mm = [ctx.medium(shape, bounds, ma, ms, g) for ma, ms in maps]
+    med_list = [m for m in mm for _ in range(args.beams)]
+    beam_list = [b for _ in mm for b in beams5]
+    nsolve = len(med_list)
+    prm = rte.make_params(L, precision=rte.PREC_MIXED, stop_rule=rte.STOP_DELTA, tol=1e-6, max_terms=30,
+                          path_mode=rte.PATH_CACHED)
+
+    def step():
+        for m in mm:
+            m.reset_cache()           # every step rebuilds the per-medium path tables
+        return ctx.solve_media(med_list, beam_list, prm)
+
+    for i in range(args.warmup):
+        log("warmup step", i)
+        step()
+    ctx.profile(True)
+    lat, times, imgs, stats = [], [], None, None
+    for i in range(args.steps):
+        for m in mm:
+            m.reset_cache()
+        t0 = time.perf_counter()
+        imgs, stats = ctx.solve_media(med_list, beam_list, prm)
+        dt = time.perf_counter() - t0
+        times.append(dt)
+        lat += [s["latency_ms"] for s in stats]
+        log("step %d: %.3f s, terms %s" % (i, dt, sorted(set(s["terms"] for s in stats))))
+    prof = ctx.get_profile()
+    ctx.profile(False)
+    ms_per_step = 1000.0 * float(np.mean(times))
+    value = nsolve / (ms_per_step / 1000.0)
+    terms_mean = float(np.mean([s["terms"] for s in stats]))
+
+    # roofline of the dominant kernel (propagation): algorithmic flops / launch time
+    launches = max(prof["prop_launches"], 1)
+    avg_ms = prof["prop_ms"] / launches
+    flops_per_launch = prof["prop_flops"] / launches
+    achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
+    pairs_per_s = prof["prop_pairs"] / (prof["prop_ms"] * 1e-3) if prof["prop_ms"] > 0 else 0.0
+
+    out = {}
+    # error against an all-FP64 solve of medium 0's beams (same term cap)
+    if not args.no_check:
+        p64 = rte.make_params(L, precision=rte.PREC_FP64, stop_rule=rte.STOP_DELTA, tol=1e-6, max_terms=30,
+                              path_mode=rte.PATH_CACHED)
+        ref, _ = ctx.solve_media([mm[0]] * args.beams, beams5, p64)
+        errs = [float(np.max(np.abs(imgs[b] - ref[b])) / np.max(np.abs(ref[b]))) for b in range(args.beams)]
+        out["rel_err_vs_fp64"] = max(errs)
+        mm[0].reset_cache()
+        # the plain path: one solve at a time, FP32 throughout, path integral
+        # marched per pair in every term (the reference's structure)
+        pplain = rte.make_params(L, precision=rte.PREC_FP32, stop_rule=rte.STOP_DELTA, tol=1e-6, max_terms=30,
+                                 path_mode=rte.PATH_MARCH)
+        t0 = time.perf_counter()
+        plain_img, _ = ctx.solve_media([mm[0]], [beams5[0]], pplain)
+        t_plain = time.perf_counter() - t0
+        out["plain_fp32_solves_per_s"] = 1.0 / t_plain
+        out["plain_rel_err_vs_fp64"] = float(np.max(np.abs(plain_img[0] - ref[0])) / np.max(np.abs(ref[0])))
+        out["batched_over_plain"] = value / out["plain_fp32_solves_per_s"]
+
+    res = {
+        "metric": "RTE solves/s + p50/p99 latency at 32^3 L=5, <=1e-4 rel exitance err vs FP64",
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (fp64 running sums)",
+        "data": "synthetic (seeded lognormal media, config 5)",
+        "config": {"workload": "config5: %d media x %d beams, %d^3 voxels 0.1 cm, L=%d, g=0.9, tol 1e-6 cap 30"
+                               % (args.media, args.beams, n, L),
+                   "solves_per_step": nsolve, "grid": n, "L": L, "precision": "mixed",
+                   "path": "cached path integrals"},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "terms_mean": terms_mean,
+        "voxel_pair_propagations_per_s": pairs_per_s,
+        "roofline": {"bound": "mfma", "pipe": "valu-fp32", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "k_prop (cached path)", "avg_launch_ms": avg_ms,
+                     "flops_per_launch": flops_per_launch},
+    }
+    res.update(out)
+    res["ktab_ms_total"] = prof["ktab_ms"]
+    if not args.no_cpu:
+        log("cpu baseline")
+        res["cpu_baseline"] = cpu_baseline(shape, bounds, maps[0][0], maps[0][1], g, L, terms_mean)
+    print(json.dumps(res), flush=True)
+    for m in mm:
+        m.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -10,6 +10,7 @@
 // the host per term, where the stop rule runs in FP64.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -257,10 +258,20 @@ struct MomentSet {
     DevBuf md, mf;  // [kNMom][K][K] in double and float
 };
 
+struct PendingTiming {
+    int kind;  // 0 prop, 1 ktab, 2 other
+    hipEvent_t a, b;
+    double pairs, flops;
+};
+
 struct rte_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool prof = false;
+    rte_prof pc{};
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<PendingTiming> pending;
     std::map<std::tuple<int, double, double, double>, std::unique_ptr<MomentSet>> moments;
     // solve workspace (grow-only)
     DevBuf ws_W, ws_part, ws_pix, ws_sv, ws_beams, ws_red;
@@ -286,6 +297,85 @@ static int set_device(rte_ctx *ctx)
     return RTE_OK;
 }
 
+// ---- launch timing (only while profiling is on) ----------------------------
+static hipEvent_t pool_event(rte_ctx *ctx)
+{
+    if (!ctx->ev_pool.empty()) {
+        hipEvent_t e = ctx->ev_pool.back();
+        ctx->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+struct LaunchTimer {
+    rte_ctx *ctx;
+    int kind;
+    hipEvent_t a = nullptr;
+    LaunchTimer(rte_ctx *c, int k) : ctx(c), kind(k)
+    {
+        if (ctx->prof && (a = pool_event(ctx))) (void)hipEventRecord(a, ctx->stream);
+    }
+    void done(double pairs = 0, double flops = 0)
+    {
+        if (!a) return;
+        hipEvent_t b = pool_event(ctx);
+        if (!b) return;
+        (void)hipEventRecord(b, ctx->stream);
+        ctx->pending.push_back(PendingTiming{kind, a, b, pairs, flops});
+        a = nullptr;
+    }
+};
+
+// folds completed launch timings into the counters (stream must be idle)
+static void drain_timings(rte_ctx *ctx)
+{
+    for (auto &p : ctx->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p.a, p.b) != hipSuccess) {
+            (void)hipGetLastError();
+            ms = 0.f;
+        }
+        if (p.kind == 0) {
+            ctx->pc.prop_launches++;
+            ctx->pc.prop_ms += ms;
+            ctx->pc.prop_pairs += p.pairs;
+            ctx->pc.prop_flops += p.flops;
+        } else if (p.kind == 1) {
+            ctx->pc.ktab_launches++;
+            ctx->pc.ktab_ms += ms;
+            ctx->pc.ktab_pairs += p.pairs;
+        } else {
+            ctx->pc.other_ms += ms;
+        }
+        ctx->ev_pool.push_back(p.a);
+        ctx->ev_pool.push_back(p.b);
+    }
+    ctx->pending.clear();
+}
+
+// pairs (d != s) with r < thr on a grid, and the sum of |dx|+|dy|+|dz| (in
+// voxels) over them: the work that defines a propagation's algorithmic flops
+static void coupled_pairs(const rte_grid &g, double thr, double *pairs, double *l1)
+{
+    const double dx = (g.xmax - g.xmin) / g.nx, dy = (g.ymax - g.ymin) / g.ny, dz = (g.zmax - g.zmin) / g.nz;
+    double P = 0, S = 0;
+    for (int k = -(g.nz - 1); k <= g.nz - 1; ++k)
+        for (int j = -(g.ny - 1); j <= g.ny - 1; ++j)
+            for (int i = -(g.nx - 1); i <= g.nx - 1; ++i) {
+                if (!i && !j && !k) continue;
+                const double r2 = (i * dx) * (i * dx) + (j * dy) * (j * dy) + (k * dz) * (k * dz);
+                if (thr > 0 && !(r2 < thr * thr)) continue;
+                const double c = (double)(g.nx - std::abs(i)) * (g.ny - std::abs(j)) * (g.nz - std::abs(k));
+                P += c;
+                S += c * (std::abs(i) + std::abs(j) + std::abs(k));
+            }
+    *pairs = P;
+    *l1 = S;
+}
+
 extern "C" int rte_version(void) { return 1; }
 
 extern "C" const char *rte_last_error(void) { return g_err.c_str(); }
@@ -307,11 +397,35 @@ extern "C" int rte_ctx_create(int device, rte_ctx **out)
     return RTE_OK;
 }
 
+extern "C" int rte_ctx_profile(rte_ctx *ctx, int on)
+{
+    if (!ctx) return fail(RTE_E_ARG, "rte_ctx_profile: NULL");
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    drain_timings(ctx);
+    if (on) ctx->pc = rte_prof{};
+    ctx->prof = on != 0;
+    return RTE_OK;
+}
+
+extern "C" int rte_ctx_get_profile(rte_ctx *ctx, rte_prof *out)
+{
+    if (!ctx || !out) return fail(RTE_E_ARG, "rte_ctx_get_profile: NULL");
+    (void)hipSetDevice(ctx->device);
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    drain_timings(ctx);
+    *out = ctx->pc;
+    return RTE_OK;
+}
+
 extern "C" int rte_ctx_destroy(rte_ctx *ctx)
 {
     if (!ctx) return RTE_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    drain_timings(ctx);
+    for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+    ctx->ev_pool.clear();
     ctx->moments.clear();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -489,7 +603,9 @@ static int ensure_ktab(rte_medium *m, double thr)
     const int clen = (g.N + nchunk - 1) / nchunk;
     const T *mut = sizeof(T) == 4 ? (const T *)m->mut_f.p : (const T *)m->mut_d.p;
     const T thr2 = thr > 0 ? T(thr * thr) : T(sizeof(T) == 4 ? 1e30 : 1e300);
+    LaunchTimer tm(m->ctx, 1);
     HIP_TRY(launch_ktab<T>(g, (g.N + clen - 1) / clen, clen, thr2, mut, tab.as<T>(), m->ctx->stream));
+    tm.done((double)g.N * g.N);
     tthr = thr;
     return RTE_OK;
 }
@@ -544,9 +660,17 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
     const int max_terms = p->max_terms > 0 ? p->max_terms : 1000;
     hipStream_t st = ctx->stream;
 
+    const auto t_start = std::chrono::steady_clock::now();
     MomentSet *mom = nullptr;
     int rc;
     if ((rc = get_moments(ctx, L, dx, dy, dz, &mom))) return rc;
+    double cp_pairs = 0, cp_l1 = 0;
+    coupled_pairs(grid, thr, &cp_pairs, &cp_l1);
+    // algorithmic flops per pair: 4K for the projection into and out of the
+    // pair direction, 16 for r^2, rsqrt, 1/r^2 and scaling, plus 2 per voxel
+    // crossed when the path integral is marched in the term
+    const double flops_per_solve = cp_pairs * (4.0 * K + 16.0) + (path == RTE_PATH_MARCH ? 2.0 * cp_l1 : 0.0);
+    std::vector<double> lat(nb, 0.0);
 
     HIP_TRY(hipEventRecord(ctx->ev0, st));
     if (path == RTE_PATH_CACHED)
@@ -601,11 +725,15 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
     SolveDev<T, TA> *svd = ctx->ws_sv.as<SolveDev<T, TA>>();
     HIP_TRY(hipMemcpyAsync(svd, sv.data(), sizeof(SolveDev<T, TA>) * nb, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(ctx->ws_beams.p, bd.data(), sizeof(BeamDev) * nb, hipMemcpyHostToDevice, st));
-    HIP_TRY(launch_fill<T>(ctx->ws_W.as<T>(), szW * nb, T(0), st));
     const Scales sc = make_scales(L);
-    HIP_TRY(launch_source<T, TA>(L, g, dx, dy, dz, grid.xmin, grid.ymin, sc, ctx->ws_beams.as<BeamDev>(),
-                                 svd, nb, st));
-    HIP_TRY(launch_bottom<T, TA>(P, svd, nb, st));
+    {
+        LaunchTimer tm(ctx, 2);
+        HIP_TRY(launch_fill<T>(ctx->ws_W.as<T>(), szW * nb, T(0), st));
+        HIP_TRY(launch_source<T, TA>(L, g, dx, dy, dz, grid.xmin, grid.ymin, sc, ctx->ws_beams.as<BeamDev>(),
+                                     svd, nb, st));
+        HIP_TRY(launch_bottom<T, TA>(P, svd, nb, st));
+        tm.done();
+    }
     ctx->host_red.resize(sizeof(TA) * 2 * nb);
     TA *red = reinterpret_cast<TA *>(ctx->host_red.data());
     HIP_TRY(hipMemcpyAsync(red, ctx->ws_red.p, sizeof(TA) * 2 * nb, hipMemcpyDeviceToHost, st));
@@ -661,16 +789,27 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
         }
         const int na = (int)act.size();
         HIP_TRY(hipMemcpyAsync(svd, svact.data(), sizeof(SolveDev<T, TA>) * na, hipMemcpyHostToDevice, st));
-        HIP_TRY(launch_prop<T, TA>(L, path, pa, svd, na, st));
-        HIP_TRY(launch_combine<T, TA>(L, ca, svd, na, st));
-        HIP_TRY(launch_bottom<T, TA>(P, svd, na, st));
+        {
+            LaunchTimer tm(ctx, 0);
+            HIP_TRY(launch_prop<T, TA>(L, path, pa, svd, na, st));
+            tm.done(cp_pairs * na, flops_per_solve * na);
+        }
+        {
+            LaunchTimer tm(ctx, 2);
+            HIP_TRY(launch_combine<T, TA>(L, ca, svd, na, st));
+            HIP_TRY(launch_bottom<T, TA>(P, svd, na, st));
+            tm.done();
+        }
         std::vector<TA> r(2 * nb);
         HIP_TRY(hipMemcpyAsync(r.data(), ctx->ws_red.p, sizeof(TA) * 2 * nb, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
+        const double now_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
         for (int j = 0; j < na; ++j) {
             HostSolve &h = hs[act[j]];
             const double dT = (double)r[2 * act[j]], sumW = (double)r[2 * act[j] + 1];
             h.T += dT;
+            lat[act[j]] = now_ms;
             if (h.state == 2) {
                 h.state = 3;
                 continue;
@@ -720,7 +859,14 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
             s.total_exitance = hs[i].T;
             s.ballistic_total = hs[i].ball;
             s.gpu_ms = ms;
+            s.latency_ms = lat[i];
         }
+    }
+    if (ctx->prof) {
+        ctx->pc.path_mode = path;
+        ctx->pc.L = L;
+        ctx->pc.precision = sizeof(T) == 8 ? RTE_PREC_FP64 : (sizeof(TA) == 8 ? RTE_PREC_MIXED : RTE_PREC_FP32);
+        drain_timings(ctx);
     }
     return RTE_OK;
 }

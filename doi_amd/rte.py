@@ -53,7 +53,14 @@ class Params(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("terms", C.c_int), ("lambda_last", C.c_double), ("rel_change_last", C.c_double),
                 ("total_exitance", C.c_double), ("ballistic_total", C.c_double),
-                ("gpu_ms", C.c_double)]
+                ("gpu_ms", C.c_double), ("latency_ms", C.c_double)]
+
+
+class Prof(C.Structure):
+    _fields_ = [("prop_launches", C.c_longlong), ("prop_ms", C.c_double), ("prop_pairs", C.c_double),
+                ("prop_flops", C.c_double), ("ktab_launches", C.c_longlong), ("ktab_ms", C.c_double),
+                ("ktab_pairs", C.c_double), ("other_ms", C.c_double), ("path_mode", C.c_int),
+                ("L", C.c_int), ("precision", C.c_int)]
 
 
 # every entry point of include/rte.h: name -> (restype, argtypes)
@@ -65,6 +72,8 @@ SIGNATURES = {
     "rte_last_error": (C.c_char_p, []),
     "rte_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
     "rte_ctx_destroy": (C.c_int, [_vp]),
+    "rte_ctx_profile": (C.c_int, [_vp, C.c_int]),
+    "rte_ctx_get_profile": (C.c_int, [_vp, C.POINTER(Prof)]),
     "rte_medium_create": (C.c_int, [_vp, C.POINTER(Grid), C.POINTER(Optics), C.POINTER(_vp)]),
     "rte_medium_destroy": (C.c_int, [_vp]),
     "rte_medium_reset_cache": (C.c_int, [_vp]),
@@ -167,6 +176,14 @@ class Context:
     def __exit__(self, *a):
         self.close()
 
+    def profile(self, on=True):
+        _check(lib().rte_ctx_profile(self.h, 1 if on else 0))
+
+    def get_profile(self):
+        p = Prof()
+        _check(lib().rte_ctx_get_profile(self.h, C.byref(p)))
+        return {k: getattr(p, k) for k, _ in Prof._fields_}
+
     def medium(self, shape, bounds, mu_a, mu_s, g, n=1.0, c=2.9979e10):
         return Medium(self, shape, bounds, mu_a, mu_s, g, n, c)
 
@@ -180,7 +197,7 @@ class Context:
         _check(lib().rte_solve_media(self.h, nb, arr, barr, C.byref(params), _p(img), st))
         return img.reshape(nb, ny, nx), [dict(terms=s.terms, lam=s.lambda_last, rel=s.rel_change_last,
                                               total=s.total_exitance, ballistic=s.ballistic_total,
-                                              gpu_ms=s.gpu_ms) for s in st]
+                                              gpu_ms=s.gpu_ms, latency_ms=s.latency_ms) for s in st]
 
     def solve(self, shape, bounds, mu_a, mu_s, g, beam, params, n=1.0, c=2.9979e10):
         m = self.medium(shape, bounds, mu_a, mu_s, g, n, c)

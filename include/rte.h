@@ -104,8 +104,23 @@ typedef struct {
     double rel_change_last; /* last relative change of total exitance */
     double total_exitance;  /* sum of the image incl. ballistic */
     double ballistic_total; /* sum of the ballistic image */
-    double gpu_ms;          /* device time of the solve (events on the solve stream) */
+    double gpu_ms;          /* device time of the whole call (events on the solve stream) */
+    double latency_ms;      /* host time from the call's start until this solve's result was final */
 } rte_stats;
+
+/* Kernel timing of a context, collected with HIP events on the solve stream
+ * around every launch while profiling is on (rte_ctx_profile). */
+typedef struct {
+    long long prop_launches; /* propagation launches (all solves of a term share one) */
+    double prop_ms;          /* summed device time of those launches */
+    double prop_pairs;       /* voxel pairs (d != s, r < prop_thresh) x solves processed */
+    double prop_flops;       /* algorithmic FP flops of those pairs (see DESIGN: F_pair) */
+    long long ktab_launches; /* path-table launches */
+    double ktab_ms;
+    double ktab_pairs;
+    double other_ms;         /* combine, bottom, source launches */
+    int path_mode, L, precision; /* of the last propagation launch */
+} rte_prof;
 
 typedef struct rte_ctx rte_ctx;
 typedef struct rte_medium rte_medium;
@@ -113,6 +128,9 @@ typedef struct rte_medium rte_medium;
 /* ---- context ------------------------------------------------------------ */
 int rte_ctx_create(int device, rte_ctx **out);
 int rte_ctx_destroy(rte_ctx *ctx);
+/* on != 0: time every launch from now on (resets the counters); 0: stop */
+int rte_ctx_profile(rte_ctx *ctx, int on);
+int rte_ctx_get_profile(rte_ctx *ctx, rte_prof *out);
 const char *rte_last_error(void);
 int rte_version(void);
 

@@ -63,6 +63,8 @@ def lib():
         L.orc_tau.restype = C.c_double
         L.orc_source.argtypes = [C.POINTER(Grid), C.POINTER(Optics), C.POINTER(Beam), C.c_int, dp, dp]
         L.orc_propagate.argtypes = [C.POINTER(Grid), C.POINTER(Optics), C.c_int, C.c_double, dp, dp]
+        L.orc_propagate_range.argtypes = [C.POINTER(Grid), C.POINTER(Optics), C.c_int, C.c_double, dp, dp,
+                                          C.c_int, C.c_int]
         L.orc_scatter.argtypes = [C.POINTER(Grid), C.POINTER(Optics), C.c_int, dp]
         L.orc_exitance.argtypes = [C.POINTER(Grid), C.POINTER(Optics), C.c_int, C.c_int, dp, dp]
         L.orc_quad_weights.argtypes = [C.c_int, C.c_int, dp]
@@ -132,6 +134,13 @@ class Problem:
         W = _f64(W)
         out = np.zeros_like(W)
         lib().orc_propagate(C.byref(self.grid), C.byref(self.optics), L, prop_thresh, _p(W), _p(out))
+        return out
+
+    def propagate_range(self, W, L, d_begin, d_end, prop_thresh=1e30):
+        W = _f64(W)
+        out = np.zeros_like(W)
+        lib().orc_propagate_range(C.byref(self.grid), C.byref(self.optics), L, prop_thresh, _p(W), _p(out),
+                                  d_begin, d_end)
         return out
 
     def scatter(self, W, L):

@@ -412,6 +412,12 @@ static int is_uniform(const double *x, int n)
 int orc_propagate(const orc_grid *g, const orc_optics *o, int L, double prop_thresh, const double *W,
                   double *out)
 {
+    return orc_propagate_range(g, o, L, prop_thresh, W, out, 0, g->nx * g->ny * g->nz);
+}
+
+int orc_propagate_range(const orc_grid *g, const orc_optics *o, int L, double prop_thresh,
+                        const double *W, double *out, int d_begin, int d_end)
+{
     geo G;
     geo_init(&G, g);
     int K = ncoef(L), N = G.N;
@@ -459,7 +465,7 @@ int orc_propagate(const orc_grid *g, const orc_optics *o, int L, double prop_thr
     }
 
 #pragma omp parallel for schedule(dynamic, 4)
-    for (int d = 0; d < N; ++d) {
+    for (int d = d_begin; d < d_end; ++d) {
         int di = d % G.nx, dj = (d / G.nx) % G.ny, dk = d / (G.nx * G.ny);
         double acc[64];
         for (int a = 0; a < K; ++a) acc[a] = 0.0;

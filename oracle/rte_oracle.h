@@ -83,6 +83,9 @@ int orc_source(const orc_grid *g, const orc_optics *o, const orc_beam *b, int L,
 /* A (propagation incl. self term and dV) on a K*N field */
 int orc_propagate(const orc_grid *g, const orc_optics *o, int L, double prop_thresh,
                   const double *W, double *out);
+/* A restricted to destination voxels d_begin <= d < d_end (other rows of out untouched) */
+int orc_propagate_range(const orc_grid *g, const orc_optics *o, int L, double prop_thresh,
+                        const double *W, double *out, int d_begin, int d_end);
 /* D (Henyey-Greenstein scatter) in place on a K*N field */
 void orc_scatter(const orc_grid *g, const orc_optics *o, int L, double *W);
 /* exitance of a field on the bottom face, without the ballistic term */
