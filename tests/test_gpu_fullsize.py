@@ -1,0 +1,73 @@
+"""GPU parity at the workloads' full sizes: config 2 (16^3, L=5) against the
+committed FP64 oracle fixture, and size-independent properties at 32^3
+(mixed vs FP64 error bar, mirror symmetry, linearity in the beam power,
+batch independence)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from doi_amd import media, rte
+from tests.helpers import rel_max
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def config2_fixture():
+    with open(os.path.join(HERE, "golden", "oracle_config2.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("prec,tol", [(rte.PREC_FP64, 1e-9), (rte.PREC_MIXED, 1e-5), (rte.PREC_FP32, 1e-4)])
+def test_config2_against_oracle_fixture(ctx, config2_fixture, prec, tol):
+    fx = config2_fixture
+    c = media.config2(seed=fx["seed"])
+    for case in fx["cases"]:
+        print("config2", case["name"], prec, flush=True)
+        p = rte.make_params(fx["L"], precision=prec, stop_rule=case["stop_rule"], tol=case["tol"],
+                            max_terms=case["max_terms"])
+        img, st = ctx.solve(c["shape"], c["bounds"], c["mu_a"], c["mu_s"], c["g"], c["beam"], p)
+        assert st["terms"] == case["terms"]
+        assert rel_max(img, np.array(case["exitance"])) < tol
+
+
+def test_32cube_mixed_within_1e4_of_fp64(ctx):
+    n, L = 32, 5
+    ma, ms = media.config5_media(1, seed=5, n=n)[0]
+    m = ctx.medium((n, n, n), media.cube_bounds(n), ma, ms, 0.9)
+    beams = media.config5_beams(seed=5, n=n)[:2]
+    out = {}
+    for prec in (rte.PREC_FP64, rte.PREC_MIXED, rte.PREC_FP32):
+        print("32^3", prec, flush=True)
+        p = rte.make_params(L, precision=prec, stop_rule=rte.STOP_DELTA, tol=1e-6, max_terms=12)
+        out[prec] = ctx.solve_media([m, m], beams, p)
+    for prec in (rte.PREC_MIXED, rte.PREC_FP32):
+        for b in range(2):
+            assert out[prec][1][b]["terms"] == out[rte.PREC_FP64][1][b]["terms"]
+            assert rel_max(out[prec][0][b], out[rte.PREC_FP64][0][b]) < 1e-4
+    m.close()
+
+
+def test_32cube_symmetry_linearity_batch(ctx):
+    n, L = 32, 3
+    # x- and y-mirror-symmetric heterogeneous medium, centred beam (4 columns)
+    rng = np.random.default_rng(4)
+    q = 0.7 + 0.6 * rng.random((n, n // 2, n // 2))
+    full = np.concatenate([q, q[:, :, ::-1]], axis=2)
+    full = np.concatenate([full, full[:, ::-1, :]], axis=1).ravel()
+    bounds = media.cube_bounds(n)
+    m = ctx.medium((n, n, n), bounds, 0.01 * full, full, 0.85)
+    other = ctx.medium((n, n, n), bounds, *media.heterogeneous(n, 9, sigma=0.3), 0.85)
+    p = rte.make_params(L, precision=rte.PREC_MIXED, stop_rule=rte.STOP_DELTA, tol=1e-6, max_terms=8)
+    imgs, st = ctx.solve_media([m, m, other], [(0, 0, 0.2, 1.0), (0, 0, 0.2, 2.5), (0.35, -0.25, 0.1, 1.0)], p)
+    e = imgs[0]
+    assert rel_max(e[:, ::-1], e) < 1e-5
+    assert rel_max(e[::-1, :], e) < 1e-5
+    assert rel_max(imgs[1], 2.5 * e) < 1e-6
+    alone, _ = ctx.solve_media([other], [(0.35, -0.25, 0.1, 1.0)], p)
+    assert np.array_equal(alone[0], imgs[2])
+    m.close()
+    other.close()
