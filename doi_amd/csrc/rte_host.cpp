@@ -17,6 +17,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <set>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -275,6 +276,7 @@ struct rte_ctx {
     std::map<std::tuple<int, double, double, double>, std::unique_ptr<MomentSet>> moments;
     // solve workspace (grow-only)
     DevBuf ws_W, ws_part, ws_pix, ws_sv, ws_beams, ws_red;
+    std::set<rte_medium *> media;  // live media of this context
     std::vector<char> host_red;
 };
 
@@ -424,6 +426,8 @@ extern "C" int rte_ctx_destroy(rte_ctx *ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     drain_timings(ctx);
+    // media still attached become invalid with their context
+    for (rte_medium *m : std::vector<rte_medium *>(ctx->media.begin(), ctx->media.end())) rte_medium_destroy(m);
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     ctx->ev_pool.clear();
     ctx->moments.clear();
@@ -516,6 +520,7 @@ extern "C" int rte_medium_create(rte_ctx *ctx, const rte_grid *grid, const rte_o
     HIP_TRY(hipMemcpy(m->mut_f.p, mt_f.data(), N * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(m->mus_f.p, ms_f.data(), N * 4, hipMemcpyHostToDevice));
     *out = m.release();
+    ctx->media.insert(*out);
     return RTE_OK;
 }
 
@@ -524,6 +529,7 @@ extern "C" int rte_medium_destroy(rte_medium *m)
     if (!m) return RTE_OK;
     (void)hipSetDevice(m->ctx->device);
     (void)hipStreamSynchronize(m->ctx->stream);
+    m->ctx->media.erase(m);
     delete m;
     return RTE_OK;
 }
@@ -677,7 +683,9 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
         for (int i = 0; i < nb; ++i)
             if ((rc = ensure_ktab<T>(media[i], thr))) return rc;
 
-    const int nchunk = pick_chunks(g.nblk, nb, N);
+    // the source chunking depends on the grid only, so a solve's partial sums,
+    // and so its result bits, are the same alone and in any batch
+    const int nchunk = pick_chunks(g.nblk, 1, N);
     const int clen = (N + nchunk - 1) / nchunk;
     const int nch = (N + clen - 1) / clen;
     const size_t szW = (size_t)N * K, szPart = (size_t)nch * N * K;

@@ -7,6 +7,7 @@ coefficient-major ``W[a, v]`` in the real SH basis, ``v = x + nx*(y + ny*z)``.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -142,6 +143,7 @@ class Medium:
         self.optics = Optics(_p(self.mu_a), _p(self.mu_s), g, n, c)
         self.h = _vp()
         _check(lib().rte_medium_create(ctx.h, C.byref(self.grid), C.byref(self.optics), C.byref(self.h)))
+        ctx._media.add(self)
 
     def reset_cache(self):
         _check(lib().rte_medium_reset_cache(self.h))
@@ -163,10 +165,13 @@ class Context:
 
     def __init__(self, device=0):
         self.h = _vp()
+        self._media = weakref.WeakSet()
         _check(lib().rte_ctx_create(device, C.byref(self.h)))
 
     def close(self):
         if self.h:
+            for m in list(self._media):   # media die with their context
+                m.close()
             lib().rte_ctx_destroy(self.h)
             self.h = _vp()
 

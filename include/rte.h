@@ -127,6 +127,7 @@ typedef struct rte_medium rte_medium;
 
 /* ---- context ------------------------------------------------------------ */
 int rte_ctx_create(int device, rte_ctx **out);
+/* also destroys the media still attached to the context (their handles become invalid) */
 int rte_ctx_destroy(rte_ctx *ctx);
 /* on != 0: time every launch from now on (resets the counters); 0: stop */
 int rte_ctx_profile(rte_ctx *ctx, int on);
