@@ -67,6 +67,7 @@ def main():
     ap.add_argument("--L", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--group", type=int, default=0, help="max beams per propagation group (0 = library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (independent solves "
@@ -83,6 +84,8 @@ def main():
     maps = media.config5_media(args.media, seed=5, n=n)
     beams5 = media.config5_beams(seed=5, n=n, per_medium=args.beams)
     ctx = rte.Context(0)
+    if args.group:
+        ctx.set_group_size(args.group)
     mm = [ctx.medium(shape, bounds, ma, ms, g) for ma, ms in maps]
     med_list = [m for m in mm for _ in range(args.beams)]
     beam_list = [b for _ in mm for b in beams5]

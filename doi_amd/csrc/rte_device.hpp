@@ -1,0 +1,206 @@
+// rte_device.hpp -- device helpers shared by the forward and adjoint kernels.
+#pragma once
+
+#include "rte_internal.hpp"
+
+namespace rte {
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__device__ __forceinline__ T dev_rsqrt(T x);
+template <>
+__device__ __forceinline__ float dev_rsqrt<float>(float x) { return __frsqrt_rn(x); }
+template <>
+__device__ __forceinline__ double dev_rsqrt<double>(double x) { return 1.0 / sqrt(x); }
+
+template <typename T>
+__device__ __forceinline__ T dev_exp(T x);
+template <>
+__device__ __forceinline__ float dev_exp<float>(float x) { return expf(x); }
+template <>
+__device__ __forceinline__ double dev_exp<double>(double x) { return exp(x); }
+
+template <typename T>
+__device__ __forceinline__ T big_value();
+template <>
+__device__ __forceinline__ float big_value<float>() { return 1.0e30f; }
+template <>
+__device__ __forceinline__ double big_value<double>() { return 1.0e300; }
+
+
+// ---------------------------------------------------------------------------
+// Siddon path integral between the centres of source s = (si,sj,sk) and
+// destination d = s + (ix,iy,iz):  tau = r * sum_v mu_t(v) dalpha(v).
+// Crossings along an axis with n steps sit at alpha = (j + 1/2)/n; ties go to
+// z, then y, then x; the destination contributes the half step 0.5/n of the
+// axis it was entered through (GPU package/src/kernels.cu:49-145).
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T march_tau(const __attribute__((address_space(1))) T *mut, int v, int ix, int iy, int iz,
+                                       int nx, int nxy, T r)
+{
+    const int n0 = abs(ix), n1 = abs(iy), n2 = abs(iz);
+    const T BIG = big_value<T>();
+    const T inc0 = n0 ? T(1) / T(n0) : BIG;
+    const T inc1 = n1 ? T(1) / T(n1) : BIG;
+    const T inc2 = n2 ? T(1) / T(n2) : BIG;
+    T a0 = T(0.5) * inc0, a1 = T(0.5) * inc1, a2 = T(0.5) * inc2;
+    const int st0 = ix > 0 ? 1 : -1, st1 = iy > 0 ? nx : -nx, st2 = iz > 0 ? nxy : -nxy;
+    const int nsteps = n0 + n1 + n2;
+    T prev = T(0), acc = T(0), last = T(0);
+    for (int t = 0; t < nsteps; ++t) {
+        const bool pz = (a2 <= a0) && (a2 <= a1);
+        const bool py = !pz && (a1 <= a0);
+        const T a = pz ? a2 : (py ? a1 : a0);
+        acc += mut[v] * (a - prev);
+        prev = a;
+        v += pz ? st2 : (py ? st1 : st0);
+        last = pz ? inc2 : (py ? inc1 : inc0);
+        a2 = pz ? a2 + inc2 : a2;
+        a1 = py ? a1 + inc1 : a1;
+        a0 = (!pz && !py) ? a0 + inc0 : a0;
+    }
+    acc += mut[v] * T(0.5) * last;
+    return acc * r;
+}
+
+// ---------------------------------------------------------------------------
+// One (destination, source) pair in the factored real-SH form, on scaled
+// coefficients (see ShConst):  p = sum_a Rt_a(u) w'_a ;  acc' += (kf p) Rt(u)
+// with Rt_l0 = Pt_l^0, Rt_lm = Pt_l^m C_m, Rt_l,-m = Pt_l^m S_m,
+// C_m + i S_m = (x + i y)^m.  The direction part (pair_geom) is shared by all
+// right-hand sides that use the same pair (beams on one medium).
+// ---------------------------------------------------------------------------
+template <typename T, int L>
+struct PairGeom {
+    T C[L + 1], S[L + 1];
+    T P[L + 1][L + 1];  // only l >= m used
+};
+
+template <typename T, int L>
+__device__ __forceinline__ void pair_geom(PairGeom<T, L> &G, T x, T y, T z, const ShConst<T> &sh)
+{
+    G.C[0] = T(1);
+    G.S[0] = T(0);
+    if constexpr (L >= 1) {
+        G.C[1] = x;
+        G.S[1] = y;
+    }
+#pragma unroll
+    for (int m = 2; m <= L; ++m) {
+        G.C[m] = G.C[m - 1] * x - G.S[m - 1] * y;
+        G.S[m] = G.S[m - 1] * x + G.C[m - 1] * y;
+    }
+#pragma unroll
+    for (int m = 0; m <= L; ++m) {
+        G.P[m][m] = T(1);
+        if (m + 1 <= L) G.P[m + 1][m] = z;
+#pragma unroll
+        for (int l = m + 2; l <= L; ++l) G.P[l][m] = z * G.P[l - 1][m] - sh.gamma[l][m] * G.P[l - 2][m];
+    }
+}
+
+// p = sum_a Rt_a(u) w'_a
+template <typename T, int L>
+__device__ __forceinline__ T pair_dot(const T (&w)[(L + 1) * (L + 1)], const PairGeom<T, L> &G)
+{
+    T p = w[0];
+#pragma unroll
+    for (int l = 1; l <= L; ++l) p += G.P[l][0] * w[l * l + l];
+#pragma unroll
+    for (int m = 1; m <= L; ++m) {
+        T A = w[m * m + m + m], B = w[m * m + m - m];
+#pragma unroll
+        for (int l = m + 1; l <= L; ++l) {
+            A += G.P[l][m] * w[l * l + l + m];
+            B += G.P[l][m] * w[l * l + l - m];
+        }
+        p += G.C[m] * A + G.S[m] * B;
+    }
+    return p;
+}
+
+// acc' += t Rt(u)
+template <typename T, int L>
+__device__ __forceinline__ void pair_axpy(T (&acc)[(L + 1) * (L + 1)], T t, const PairGeom<T, L> &G)
+{
+    acc[0] += t;
+#pragma unroll
+    for (int l = 1; l <= L; ++l) acc[l * l + l] += t * G.P[l][0];
+#pragma unroll
+    for (int m = 1; m <= L; ++m) {
+        const T u = t * G.C[m], vv = t * G.S[m];
+        acc[m * m + m + m] += u;
+        acc[m * m + m - m] += vv;
+#pragma unroll
+        for (int l = m + 1; l <= L; ++l) {
+            acc[l * l + l + m] += u * G.P[l][m];
+            acc[l * l + l - m] += vv * G.P[l][m];
+        }
+    }
+}
+
+template <typename T, int L>
+__device__ __forceinline__ void pair_apply(T (&acc)[(L + 1) * (L + 1)], const T (&w)[(L + 1) * (L + 1)], T kf,
+                                           const PairGeom<T, L> &G)
+{
+    pair_axpy<T, L>(acc, kf * pair_dot<T, L>(w, G), G);
+}
+
+// Walks the same Siddon crossings as march_tau and calls f(v, length) for
+// every voxel v crossed (length = r * dalpha, the destination's half step
+// included): the transpose of the path integral, used to back-project.
+template <typename T, typename F>
+__device__ __forceinline__ void march_visit(int v, int ix, int iy, int iz, int nx, int nxy, T r, F &&f)
+{
+    const int n0 = abs(ix), n1 = abs(iy), n2 = abs(iz);
+    const T BIG = big_value<T>();
+    const T inc0 = n0 ? T(1) / T(n0) : BIG;
+    const T inc1 = n1 ? T(1) / T(n1) : BIG;
+    const T inc2 = n2 ? T(1) / T(n2) : BIG;
+    T a0 = T(0.5) * inc0, a1 = T(0.5) * inc1, a2 = T(0.5) * inc2;
+    const int st0 = ix > 0 ? 1 : -1, st1 = iy > 0 ? nx : -nx, st2 = iz > 0 ? nxy : -nxy;
+    const int nsteps = n0 + n1 + n2;
+    T prev = T(0), last = T(0);
+    for (int t = 0; t < nsteps; ++t) {
+        const bool pz = (a2 <= a0) && (a2 <= a1);
+        const bool py = !pz && (a1 <= a0);
+        const T a = pz ? a2 : (py ? a1 : a0);
+        f(v, r * (a - prev));
+        prev = a;
+        v += pz ? st2 : (py ? st1 : st0);
+        last = pz ? inc2 : (py ? inc1 : inc0);
+        a2 = pz ? a2 + inc2 : a2;
+        a1 = py ? a1 + inc1 : a1;
+        a0 = (!pz && !py) ? a0 + inc0 : a0;
+    }
+    f(v, r * T(0.5) * last);
+}
+
+// address-space views: wave-uniform reads through the scalar cache, per-lane
+// reads as global (not flat) loads
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T *;
+template <typename T>
+using gptr = const __attribute__((address_space(1))) T *;
+template <typename T>
+__device__ __forceinline__ cptr<T> as_const(const T *p) { return (cptr<T>)(p); }
+template <typename T>
+__device__ __forceinline__ gptr<T> as_global(const T *p) { return (gptr<T>)(p); }
+
+// destination of a lane: brick blk, lane -> (di, dj, dk)
+template <typename T>
+__device__ __forceinline__ void lane_dest(const GridDev<T> &g, int blk, int lane, int &di, int &dj,
+                                          int &dk)
+{
+    const int bxi = blk % g.bx, byi = (blk / g.bx) % g.by, bzi = blk / (g.bx * g.by);
+    di = bxi * kBrick + (lane & 3);
+    dj = byi * kBrick + ((lane >> 2) & 3);
+    dk = bzi * kBrick + (lane >> 4);
+}
+
+
+}  // namespace rte

@@ -15,6 +15,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <map>
 #include <memory>
 #include <set>
@@ -270,6 +271,7 @@ struct rte_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool prof = false;
+    int max_rhs = kMaxNR;
     rte_prof pc{};
     std::vector<hipEvent_t> ev_pool;
     std::vector<PendingTiming> pending;
@@ -396,6 +398,14 @@ extern "C" int rte_ctx_create(int device, rte_ctx **out)
     HIP_TRY(hipEventCreate(&ctx->ev0));
     HIP_TRY(hipEventCreate(&ctx->ev1));
     *out = ctx.release();
+    return RTE_OK;
+}
+
+extern "C" int rte_ctx_set_group_size(rte_ctx *ctx, int max_rhs)
+{
+    if (!ctx) return fail(RTE_E_ARG, "rte_ctx_set_group_size: NULL");
+    if (max_rhs < 1 || max_rhs > kMaxNR) return fail(RTE_E_ARG, "group size %d outside 1..%d", max_rhs, kMaxNR);
+    ctx->max_rhs = max_rhs;
     return RTE_OK;
 }
 
@@ -651,6 +661,39 @@ static double stop_weight(int a)
     return 0.0;
 }
 
+// One term of the Neumann series has been propagated: fold its exitance
+// increment and stop-rule sum into the series state and decide what comes
+// next (0 another term, 2 the final exitance-only propagation, 3 done).
+static void advance_series(HostSolve &h, double dT, double sumW, int rule, double tol, int max_terms)
+{
+    h.T += dT;
+    if (h.state == 2) {
+        h.state = 3;
+        return;
+    }
+    h.n += 1;
+    h.rel = h.T != 0.0 ? std::fabs(dT) / std::fabs(h.T) : 0.0;
+    if (rule == RTE_STOP_DELTA) {
+        if (h.rel < tol || h.n >= max_terms) h.state = 3;
+        return;
+    }
+    // CPU package/src/Kernels_selfsub.c:1208-1229 at n + 1
+    h.sumOut += sumW;
+    bool stop;
+    if (h.sumOut == 0.0) {
+        stop = true;
+        h.lam = 0.0;
+    } else {
+        const double gam = sumW / h.sumOut;
+        double hi = 1.0 - gam * (1.0 / (1.0 - gam / h.fi));
+        h.fi = gam;
+        if (hi > 1.0 || hi < 0.0) hi = 0.0;
+        h.lam = hi;
+        stop = hi > 0.9999;
+    }
+    if (stop || h.n >= max_terms) h.state = 2;
+}
+
 template <typename T, typename TA>
 static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_beam *beams,
                      const rte_params *p, double *exitance, rte_stats *stats)
@@ -675,7 +718,11 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
     // algorithmic flops per pair: 4K for the projection into and out of the
     // pair direction, 16 for r^2, rsqrt, 1/r^2 and scaling, plus 2 per voxel
     // crossed when the path integral is marched in the term
-    const double flops_per_solve = cp_pairs * (4.0 * K + 16.0) + (path == RTE_PATH_MARCH ? 2.0 * cp_l1 : 0.0);
+    // (beams sharing one medium in a launch share the 16 direction flops)
+    const double flops_dir = cp_pairs * 16.0 + (path == RTE_PATH_MARCH ? 2.0 * cp_l1 : 0.0);
+    const double flops_rhs = cp_pairs * 4.0 * K;
+    // beams on one medium are propagated together on the mixed cached path
+    const bool multi = sizeof(T) == 4 && sizeof(TA) == 8 && path == RTE_PATH_CACHED;
     std::vector<double> lat(nb, 0.0);
 
     HIP_TRY(hipEventRecord(ctx->ev0, st));
@@ -693,7 +740,7 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
     HIP_TRY(ctx->ws_part.alloc(szPart * sizeof(T) * nb));
     HIP_TRY(ctx->ws_pix.alloc((size_t)3 * P * sizeof(TA) * nb));
     HIP_TRY(ctx->ws_red.alloc((size_t)2 * sizeof(TA) * nb));
-    HIP_TRY(ctx->ws_sv.alloc(sizeof(SolveDev<T, TA>) * nb));
+    HIP_TRY(ctx->ws_sv.alloc(sizeof(SolveDev<T, TA>) * nb * 2));
     HIP_TRY(ctx->ws_beams.alloc(sizeof(BeamDev) * nb));
 
     std::vector<SolveDev<T, TA>> sv(nb);
@@ -784,7 +831,8 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
     }
 
     std::vector<int> act;
-    std::vector<SolveDev<T, TA>> svact;
+    std::vector<SolveDev<T, TA>> svact, svgrp;
+    SolveDev<T, TA> *svd_grp = svd + nb;
     for (;;) {
         act.clear();
         for (int i = 0; i < nb; ++i)
@@ -797,10 +845,44 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
         }
         const int na = (int)act.size();
         HIP_TRY(hipMemcpyAsync(svd, svact.data(), sizeof(SolveDev<T, TA>) * na, hipMemcpyHostToDevice, st));
-        {
+        if (!multi) {
             LaunchTimer tm(ctx, 0);
-            HIP_TRY(launch_prop<T, TA>(L, path, pa, svd, na, st));
-            tm.done(cp_pairs * na, flops_per_solve * na);
+            HIP_TRY(launch_prop<T, TA>(L, path, 1, pa, svd, na, st));
+            tm.done(cp_pairs * na, (flops_dir + flops_rhs) * na);
+        } else {
+            // groups of up to kMaxNR active solves on the same medium, launched
+            // by group size (the arithmetic of a solve does not depend on it)
+            std::vector<std::vector<int>> by_size(kMaxNR + 1);  // indices into svact
+            std::vector<char> taken(na, 0);
+            for (int j = 0; j < na; ++j) {
+                if (taken[j]) continue;
+                std::vector<int> same;
+                for (int q = j; q < na; ++q)
+                    if (!taken[q] && svact[q].ktab == svact[j].ktab) same.push_back(q);
+                const size_t gs = (size_t)ctx->max_rhs;
+                for (size_t o = 0; o < same.size(); o += gs) {
+                    const size_t e = std::min(same.size(), o + gs);
+                    for (size_t q = o; q < e; ++q) {
+                        by_size[e - o].push_back(same[q]);
+                        taken[same[q]] = 1;
+                    }
+                }
+            }
+            svgrp.clear();
+            std::vector<std::pair<int, int>> launches;  // (nr, offset)
+            for (int k = kMaxNR; k >= 1; --k) {
+                if (by_size[k].empty()) continue;
+                launches.emplace_back(k, (int)svgrp.size());
+                for (int q : by_size[k]) svgrp.push_back(svact[q]);
+            }
+            HIP_TRY(hipMemcpyAsync(svd_grp, svgrp.data(), sizeof(SolveDev<T, TA>) * svgrp.size(),
+                                   hipMemcpyHostToDevice, st));
+            for (auto &ln : launches) {
+                const int k = ln.first, ng = (int)by_size[k].size() / k;
+                LaunchTimer tm(ctx, 0);
+                HIP_TRY(launch_prop<T, TA>(L, path, k, pa, svd_grp + ln.second, ng, st));
+                tm.done(cp_pairs * k * ng, (flops_dir + k * flops_rhs) * ng);
+            }
         }
         {
             LaunchTimer tm(ctx, 2);
@@ -816,35 +898,8 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
         for (int j = 0; j < na; ++j) {
             HostSolve &h = hs[act[j]];
             const double dT = (double)r[2 * act[j]], sumW = (double)r[2 * act[j] + 1];
-            h.T += dT;
             lat[act[j]] = now_ms;
-            if (h.state == 2) {
-                h.state = 3;
-                continue;
-            }
-            if (p->stop_rule == RTE_STOP_DELTA) {
-                h.n += 1;
-                h.rel = h.T != 0.0 ? std::fabs(dT) / std::fabs(h.T) : 0.0;
-                if (h.rel < p->tol || h.n >= max_terms) h.state = 3;
-            } else {
-                // CPU package/src/Kernels_selfsub.c:1208-1229 at n + 1
-                h.sumOut += sumW;
-                h.n += 1;
-                h.rel = h.T != 0.0 ? std::fabs(dT) / std::fabs(h.T) : 0.0;
-                bool stop;
-                if (h.sumOut == 0.0) {
-                    stop = true;
-                    h.lam = 0.0;
-                } else {
-                    const double gam = sumW / h.sumOut;
-                    double hi = 1.0 - gam * (1.0 / (1.0 - gam / h.fi));
-                    h.fi = gam;
-                    if (hi > 1.0 || hi < 0.0) hi = 0.0;
-                    h.lam = hi;
-                    stop = hi > 0.9999;
-                }
-                if (stop || h.n >= max_terms) h.state = 2;
-            }
+            advance_series(h, dT, sumW, p->stop_rule, p->tol, max_terms);
         }
         // the rows of svd hold the active solves; stale entries are rebuilt next pass
     }
@@ -885,7 +940,7 @@ static int check_params(const rte_params *p)
     if (p->L < 0 || p->L > RTE_MAX_L) return fail(RTE_E_UNSUPP, "L=%d outside 0..%d", p->L, RTE_MAX_L);
     if (p->stop_rule != RTE_STOP_LAMBDA && p->stop_rule != RTE_STOP_DELTA)
         return fail(RTE_E_ARG, "unknown stop_rule %d", p->stop_rule);
-    if (p->stop_rule == RTE_STOP_DELTA && !(p->tol > 0.0)) return fail(RTE_E_ARG, "tol must be > 0");
+    if (p->stop_rule == RTE_STOP_DELTA && !(p->tol >= 0.0)) return fail(RTE_E_ARG, "tol must be >= 0");
     if (p->precision < RTE_PREC_FP64 || p->precision > RTE_PREC_FP32)
         return fail(RTE_E_ARG, "unknown precision %d", p->precision);
     if (p->exit_quad != RTE_QUAD_REF && p->exit_quad != RTE_QUAD_EXACT)
@@ -950,6 +1005,304 @@ extern "C" long long rte_solve_footprint(const rte_grid *grid, const rte_params 
 }
 
 // ---------------------------------------------------------------------------
+// reverse mode: vector-Jacobian product of the exitance (rte_adjoint.hip)
+// ---------------------------------------------------------------------------
+template <typename T, typename TA>
+static int vjp_impl(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte_params *p, int stride,
+                    const double *cot_in, const double *target, double *loss, double *exitance, double *dmu_a,
+                    double *dmu_s, rte_stats *stats)
+{
+    const rte_grid &grid = m->grid;
+    const int L = p->L, K = (L + 1) * (L + 1);
+    const GridDev<T> g = make_grid<T>(grid);
+    const int N = g.N, P = grid.nx * grid.ny;
+    const double dx = (grid.xmax - grid.xmin) / grid.nx, dy = (grid.ymax - grid.ymin) / grid.ny,
+                 dz = (grid.zmax - grid.zmin) / grid.nz, dV = dx * dy * dz;
+    const double thr = p->prop_thresh > 0 ? p->prop_thresh : -1.0;
+    const int max_terms = p->max_terms > 0 ? p->max_terms : 1000;
+    hipStream_t st = ctx->stream;
+    int rc;
+    MomentSet *mom = nullptr;
+    if ((rc = get_moments(ctx, L, dx, dy, dz, &mom))) return rc;
+    HIP_TRY(hipEventRecord(ctx->ev0, st));
+    if ((rc = ensure_ktab<T>(m, thr))) return rc;
+    const int nch0 = pick_chunks(g.nblk, 1, N);
+    const int clen = (N + nch0 - 1) / nch0;
+    const int nch = (N + clen - 1) / clen;
+    const size_t F = (size_t)N * K;
+
+    DevBuf dW, dS0, dPart, dPix, dRed, dSv, dBeam, dCot, dCoefE, dX, dWq, dPair, dGt, dGs, dGa2, dGs2, dLam0;
+    HIP_TRY(dW.alloc(F * sizeof(T)));
+    HIP_TRY(dS0.alloc(F * sizeof(T)));
+    HIP_TRY(dPart.alloc((size_t)nch * F * sizeof(T)));
+    HIP_TRY(dPix.alloc((size_t)3 * P * sizeof(TA)));
+    HIP_TRY(dRed.alloc(2 * sizeof(TA)));
+    HIP_TRY(dSv.alloc(sizeof(SolveDev<T, TA>)));
+    HIP_TRY(dBeam.alloc(sizeof(BeamDev)));
+
+    // ---- forward series, keeping A W_q at the checkpoints ----
+    const double half = (beam->diam + 1.0e-16) / 2.0;
+    int cols = 0;
+    for (int j = 0; j < grid.ny; ++j)
+        for (int i = 0; i < grid.nx; ++i) {
+            const double xc = (double)i * dx + grid.xmin + dx / 2.0, yc = (double)j * dy + grid.ymin + dy / 2.0;
+            if (std::sqrt((xc - beam->pos_x) * (xc - beam->pos_x) + (yc - beam->pos_y) * (yc - beam->pos_y)) <= half)
+                ++cols;
+        }
+    if (cols == 0) return fail(RTE_E_ARG, "beam hits no voxel column");
+    const BeamDev bd{beam->pos_x, beam->pos_y, half, beam->mag / cols, m->mua_d.as<double>(), m->mus_d.as<double>()};
+    SolveDev<T, TA> sv;
+    std::memset(&sv, 0, sizeof sv);
+    sv.W = dW.as<T>();
+    sv.part = dPart.as<T>();
+    sv.ktab = sizeof(T) == 4 ? (const T *)m->ktab_f.p : (const T *)m->ktab_d.p;
+    sv.mut = sizeof(T) == 4 ? (const T *)m->mut_f.p : (const T *)m->mut_d.p;
+    sv.mus = sizeof(T) == 4 ? (const T *)m->mus_f.p : (const T *)m->mus_d.p;
+    sv.g = T(m->g);
+    sv.dE = dPix.as<TA>();
+    sv.bsum = sv.dE + P;
+    sv.E = sv.dE + 2 * P;
+    sv.red = dRed.as<TA>();
+    sv.mode = 1;
+    SolveDev<T, TA> *svd = dSv.as<SolveDev<T, TA>>();
+    const Scales sc = make_scales(L);
+    HIP_TRY(hipMemcpyAsync(svd, &sv, sizeof sv, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(dBeam.p, &bd, sizeof bd, hipMemcpyHostToDevice, st));
+    HIP_TRY(launch_fill<T>(dW.as<T>(), F, T(0), st));
+    HIP_TRY(launch_source<T, TA>(L, g, dx, dy, dz, grid.xmin, grid.ymin, sc, dBeam.as<BeamDev>(), svd, 1, st));
+    HIP_TRY(launch_bottom<T, TA>(P, svd, 1, st));
+    HIP_TRY(hipMemcpyAsync(dS0.p, dW.p, F * sizeof(T), hipMemcpyDeviceToDevice, st));
+    TA red[2];
+    HIP_TRY(hipMemcpyAsync(red, dRed.p, sizeof red, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    HostSolve h{m, 0, 0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0};
+    h.ball = (double)red[0];
+    h.T = h.ball;
+    if (p->stop_rule == RTE_STOP_LAMBDA) {
+        h.n = 1;
+        h.sumOut = (double)red[1];
+        h.state = (max_terms <= 1) ? 2 : 0;
+    }
+
+    PropArgs<T> pa;
+    pa.g = g;
+    pa.sh = make_sh_const<T>();
+    pa.nchunk = nch;
+    pa.chunk_len = clen;
+    pa.thr2 = thr > 0 ? T(thr * thr) : T(sizeof(T) == 4 ? 1e30 : 1e300);
+    CombArgs<T, TA> ca;
+    std::memset(&ca, 0, sizeof ca);
+    ca.g = g;
+    ca.nchunk = nch;
+    ca.mom = sizeof(TA) == 4 ? (const TA *)mom->mf.p : (const TA *)mom->md.p;
+    ca.self_scale = TA((M_PI / kAngRes) * (2.0 * M_PI / kAngRes) / dV);
+    ca.dV = TA(dV);
+    double Q[kMaxL + 1], coefE[kMaxL + 1];
+    quad_weights(L, p->exit_quad, Q);
+    for (int l = 0; l <= L; ++l) {
+        coefE[l] = 2.0 * M_PI * dx * dy / std::sqrt(4.0 * M_PI) * std::sqrt(2.0 * l + 1.0) * Q[l];
+        ca.coefE[l] = TA(coefE[l]);
+    }
+    for (int a = 0; a < K; ++a) ca.wsum[a] = TA(stop_weight(a));
+    for (int a = 0; a < kMaxK; ++a) ca.nsc[a] = TA(sc.n[a]);
+
+    // checkpoints: A W_q is kept when (q + 1) % stride == 0 (all of them for stride 1)
+    std::vector<std::unique_ptr<DevBuf>> ckpt;
+    auto ckpt_slot = [&](int q) -> TA * {
+        if ((q + 1) % stride != 0) return nullptr;
+        const size_t idx = (size_t)q;
+        while (ckpt.size() <= idx) ckpt.emplace_back(nullptr);
+        if (!ckpt[idx]) {
+            ckpt[idx].reset(new DevBuf());
+            if (ckpt[idx]->alloc(F * sizeof(TA)) != hipSuccess) return nullptr;
+        }
+        return ckpt[idx]->as<TA>();
+    };
+    int q = 0;
+    while (h.state != 3) {
+        sv.mode = h.state;
+        sv.aw_store = ckpt_slot(q);
+        if ((q + 1) % stride == 0 && !sv.aw_store) return fail(RTE_E_NOMEM, "checkpoint allocation failed");
+        HIP_TRY(hipMemcpyAsync(svd, &sv, sizeof sv, hipMemcpyHostToDevice, st));
+        HIP_TRY(launch_prop<T, TA>(L, RTE_PATH_CACHED, 1, pa, svd, 1, st));
+        HIP_TRY(launch_combine<T, TA>(L, ca, svd, 1, st));
+        HIP_TRY(launch_bottom<T, TA>(P, svd, 1, st));
+        HIP_TRY(hipMemcpyAsync(red, dRed.p, sizeof red, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        advance_series(h, (double)red[0], (double)red[1], p->stop_rule, p->tol, max_terms);
+        ++q;
+    }
+    const int n = q;   // W_0 .. W_{n-1} entered the exitance
+
+    // ---- image, loss and cotangent ----
+    std::vector<TA> E(P);
+    HIP_TRY(hipMemcpy(E.data(), sv.E, sizeof(TA) * P, hipMemcpyDeviceToHost));
+    std::vector<TA> cot(P);
+    double lsum = 0.0;
+    for (int i = 0; i < P; ++i) {
+        if (exitance) exitance[i] = (double)E[i];
+        if (target) {
+            const double d = (double)E[i] - target[i];
+            lsum += d * d;
+            cot[i] = TA(2.0 * d);
+        } else {
+            cot[i] = TA(cot_in[i]);
+        }
+    }
+    if (loss) *loss = lsum;
+    HIP_TRY(dCot.alloc(sizeof(TA) * P));
+    HIP_TRY(dCoefE.alloc(sizeof(TA) * (kMaxL + 1)));
+    HIP_TRY(hipMemcpyAsync(dCot.p, cot.data(), sizeof(TA) * P, hipMemcpyHostToDevice, st));
+    TA cE[kMaxL + 1];
+    for (int l = 0; l <= kMaxL; ++l) cE[l] = l <= L ? TA(coefE[l]) : TA(0);
+    HIP_TRY(hipMemcpyAsync(dCoefE.p, cE, sizeof cE, hipMemcpyHostToDevice, st));
+
+    // ---- adjoint series ----
+    HIP_TRY(dX.alloc(F * sizeof(T)));
+    HIP_TRY(dWq.alloc(F * sizeof(T)));
+    HIP_TRY(dPair.alloc((size_t)N * g.npad * sizeof(T)));
+    HIP_TRY(dGt.alloc(N * sizeof(TA)));
+    HIP_TRY(dGs.alloc(N * sizeof(TA)));
+    HIP_TRY(dGa2.alloc(N * sizeof(TA)));
+    HIP_TRY(dGs2.alloc(N * sizeof(TA)));
+    HIP_TRY(dLam0.alloc((size_t)N * (L + 1) * sizeof(TA)));
+    HIP_TRY(launch_fill<T>(dPair.as<T>(), (size_t)N * g.npad, T(0), st));
+    HIP_TRY(launch_fill<TA>(dGt.as<TA>(), N, TA(0), st));
+    HIP_TRY(launch_fill<TA>(dGs.as<TA>(), N, TA(0), st));
+    HIP_TRY(launch_fill<TA>(dGa2.as<TA>(), N, TA(0), st));
+    HIP_TRY(launch_fill<TA>(dGs2.as<TA>(), N, TA(0), st));
+    HIP_TRY(launch_adj_init<T, TA>(L, g, dCot.as<TA>(), sc, dCoefE.as<TA>(), dX.as<T>(), st));
+
+    // the anisotropy exactly as the forward scatter used it (compute precision)
+    const TA gan = TA(T(m->g));
+    AdjCombArgs<T, TA> aa;
+    std::memset(&aa, 0, sizeof aa);
+    aa.g = g;
+    aa.nchunk = nch;
+    aa.mom = ca.mom;
+    aa.self_scale = ca.self_scale;
+    aa.dV = ca.dV;
+    for (int l = 0; l <= kMaxL; ++l) aa.coefE[l] = ca.coefE[l];
+    for (int a = 0; a < kMaxK; ++a) aa.nsc[a] = ca.nsc[a];
+    aa.X = dX.as<T>();
+    aa.Wq = dWq.as<T>();
+    aa.part = dPart.as<T>();
+    aa.mut = sv.mut;
+    aa.mus = sv.mus;
+    aa.g_aniso = gan;
+    aa.cot = dCot.as<TA>();
+    aa.gt = dGt.as<TA>();
+    aa.gs = dGs.as<TA>();
+
+    // segment buffer for recomputed A W_q between checkpoints
+    std::vector<std::unique_ptr<DevBuf>> seg;
+    for (int i = 0; i + 1 < stride; ++i) {
+        seg.emplace_back(new DevBuf());
+        HIP_TRY(seg.back()->alloc(F * sizeof(TA)));
+    }
+    const int nseg = (n + stride - 1) / stride;
+    for (int j = nseg - 1; j >= 0; --j) {
+        const int jc = j * stride, end = std::min(n, jc + stride);
+        // A W_q for q in [jc, end-2]: stored (stride 1) or recomputed from W_jc
+        auto aw_of = [&](int qq) -> const TA * {
+            if (qq < 0) return nullptr;
+            if (stride == 1 || (qq + 1) % stride == 0) return ckpt[qq]->as<TA>();
+            return seg[qq - jc]->as<TA>();
+        };
+        if (stride > 1 && end - 1 > jc) {
+            if (jc == 0)
+                HIP_TRY(hipMemcpyAsync(dW.p, dS0.p, F * sizeof(T), hipMemcpyDeviceToDevice, st));
+            else
+                HIP_TRY(launch_make_wq<T, TA>(L, g, sc, gan, sv.mus, aw_of(jc - 1), dW.as<T>(), st));
+            for (int qq = jc; qq <= end - 2; ++qq) {
+                sv.mode = 3;
+                sv.aw_store = seg[qq - jc]->as<TA>();
+                HIP_TRY(hipMemcpyAsync(svd, &sv, sizeof sv, hipMemcpyHostToDevice, st));
+                HIP_TRY(launch_prop<T, TA>(L, RTE_PATH_CACHED, 1, pa, svd, 1, st));
+                HIP_TRY(launch_combine<T, TA>(L, ca, svd, 1, st));
+            }
+        }
+        for (int qq = end - 1; qq >= jc; --qq) {
+            const TA *awp = aw_of(qq - 1);
+            if (qq == 0)
+                HIP_TRY(hipMemcpyAsync(dWq.p, dS0.p, F * sizeof(T), hipMemcpyDeviceToDevice, st));
+            else
+                HIP_TRY(launch_make_wq<T, TA>(L, g, sc, gan, sv.mus, awp, dWq.as<T>(), st));
+            HIP_TRY(launch_adj_prop<T, TA>(L, pa, dX.as<T>(), dWq.as<T>(), dPart.as<T>(), dPair.as<T>(), sv.ktab, st));
+            aa.aw_prev = awp;
+            aa.lam0 = (qq == 0) ? dLam0.as<TA>() : nullptr;
+            HIP_TRY(launch_adj_combine<T, TA>(L, aa, st));
+        }
+    }
+    HIP_TRY(launch_backproject<T, TA>(g, nch, clen, pa.thr2, dPair.as<T>(), TA(-dV), dGt.as<TA>(), st));
+    HIP_TRY(launch_src_grad<TA>(L, grid.nx, grid.ny, grid.nz, dx, dy, dz, grid.xmin, grid.ymin, m->g, bd,
+                                dLam0.as<TA>(), dCot.as<TA>(), dGa2.as<TA>(), dGs2.as<TA>(), st));
+    HIP_TRY(hipEventRecord(ctx->ev1, st));
+    std::vector<TA> gt(N), gs(N), ga2(N), gs2(N);
+    HIP_TRY(hipMemcpyAsync(gt.data(), dGt.p, sizeof(TA) * N, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(gs.data(), dGs.p, sizeof(TA) * N, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(ga2.data(), dGa2.p, sizeof(TA) * N, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(gs2.data(), dGs2.p, sizeof(TA) * N, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (int v = 0; v < N; ++v) {
+        if (dmu_a) dmu_a[v] = (double)gt[v] + (double)ga2[v];
+        if (dmu_s) dmu_s[v] = (double)gt[v] + (double)gs[v] + (double)gs2[v];
+    }
+    if (stats) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        std::memset(stats, 0, sizeof *stats);
+        stats->terms = h.n;
+        stats->lambda_last = h.lam;
+        stats->rel_change_last = h.rel;
+        stats->total_exitance = h.T;
+        stats->ballistic_total = h.ball;
+        stats->gpu_ms = ms;
+    }
+    (void)n;
+    return RTE_OK;
+}
+
+static int vjp_entry(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte_params *params,
+                     const rte_grad_opts *opts, const double *cot, const double *target, double *loss,
+                     double *exitance, double *dmu_a, double *dmu_s, rte_stats *stats)
+{
+    if (!ctx || !m || !beam || !params) return fail(RTE_E_ARG, "gradient: NULL argument");
+    if (m->ctx != ctx) return fail(RTE_E_ARG, "gradient: medium belongs to another context");
+    int rc;
+    if ((rc = check_params(params))) return rc;
+    if (params->path_mode != RTE_PATH_CACHED)
+        return fail(RTE_E_UNSUPP, "gradient: needs the cached path tables (path_mode RTE_PATH_CACHED)");
+    const int stride = opts ? opts->checkpoint_stride : 1;
+    if (stride < 1) return fail(RTE_E_ARG, "checkpoint_stride must be >= 1");
+    if ((rc = set_device(ctx))) return rc;
+    switch (params->precision) {
+    case RTE_PREC_FP64:
+        return vjp_impl<double, double>(ctx, m, beam, params, stride, cot, target, loss, exitance, dmu_a, dmu_s, stats);
+    case RTE_PREC_MIXED:
+        return vjp_impl<float, double>(ctx, m, beam, params, stride, cot, target, loss, exitance, dmu_a, dmu_s, stats);
+    default:
+        return fail(RTE_E_UNSUPP, "gradient: precision must be RTE_PREC_FP64 or RTE_PREC_MIXED");
+    }
+}
+
+extern "C" int rte_exitance_vjp(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte_params *params,
+                                const rte_grad_opts *opts, const double *cotangent, double *exitance, double *dmu_a,
+                                double *dmu_s, rte_stats *stats)
+{
+    if (!cotangent) return fail(RTE_E_ARG, "rte_exitance_vjp: cotangent is NULL");
+    return vjp_entry(ctx, m, beam, params, opts, cotangent, nullptr, nullptr, exitance, dmu_a, dmu_s, stats);
+}
+
+extern "C" int rte_gradient(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte_params *params,
+                            const rte_grad_opts *opts, const double *target, double *loss, double *exitance,
+                            double *dmu_a, double *dmu_s, rte_stats *stats)
+{
+    if (!target) return fail(RTE_E_ARG, "rte_gradient: target is NULL");
+    return vjp_entry(ctx, m, beam, params, opts, nullptr, target, loss, exitance, dmu_a, dmu_s, stats);
+}
+
+// ---------------------------------------------------------------------------
 // operators
 // ---------------------------------------------------------------------------
 template <typename T, typename TA>
@@ -996,7 +1349,7 @@ static int propagate_impl(rte_ctx *ctx, rte_medium *m, int L, double prop_thresh
     pa.nchunk = nch;
     pa.chunk_len = clen;
     pa.thr2 = thr > 0 ? T(thr * thr) : T(sizeof(T) == 4 ? 1e30 : 1e300);
-    HIP_TRY(launch_prop<T, TA>(L, path, pa, dSv.as<SolveDev<T, TA>>(), 1, st));
+    HIP_TRY(launch_prop<T, TA>(L, path, 1, pa, dSv.as<SolveDev<T, TA>>(), 1, st));
     CombArgs<T, TA> ca;
     std::memset(&ca, 0, sizeof ca);
     ca.g = g;

@@ -14,6 +14,7 @@ constexpr int kAngRes = 200;   // self-block quadrature grid (CPU package/src/Ke
 constexpr int kNOmega = kAngRes * kAngRes;
 constexpr int kBrick = 4;      // destination bricks of 4x4x4 voxels = one wave64
 constexpr int kWave = 64;
+constexpr int kMaxNR = 5;      // right-hand sides (beams on one medium) per propagation wave
 
 // Geometry in device form.  Voxel v = x + nx*(y + ny*z); destination bricks
 // are numbered bx fastest; slot = brick*64 + lane, lane = lx + 4*(ly + 4*lz).
@@ -51,7 +52,8 @@ struct SolveDev {
     TA *bsum;          // [nx*ny] per-pixel bottom sums of the next term (stop rule)
     TA *E;             // [nx*ny] accumulated exitance (ballistic included)
     TA *red;           // [2] reduced (sum dE, sum bsum) of this term
-    int mode;          // 0 term, 1 init, 2 final (see k_bottom / k_combine)
+    TA *aw_store;      // [N][K] if set, k_combine also writes A W (unscaled) here
+    int mode;          // 0 term, 1 init, 2 final, 3 recompute (see k_bottom / k_combine)
 };
 
 template <typename T>
@@ -91,8 +93,9 @@ struct BeamDev {
 // launchers (rte_kernels.hip)
 template <typename T>
 hipError_t launch_fill(T *p, size_t n, T v, hipStream_t st);
+// sv: ngroups * nr solves, group z = sv[z*nr .. z*nr+nr-1] on one medium
 template <typename T, typename TA>
-hipError_t launch_prop(int L, int path, const PropArgs<T> &A, const SolveDev<T, TA> *sv, int nsolve,
+hipError_t launch_prop(int L, int path, int nr, const PropArgs<T> &A, const SolveDev<T, TA> *sv, int ngroups,
                        hipStream_t st);
 template <typename T>
 hipError_t launch_ktab(const GridDev<T> &g, int nchunk, int chunk_len, T thr2, const T *mut, T *ktab,
@@ -119,5 +122,44 @@ hipError_t launch_exitance_op(int L, int nx, int ny, int nz, const double *coefE
                               double *img, hipStream_t st);
 hipError_t launch_tau_list(const GridDev<float> &g, const float *mut, int np, const int *dst,
                            const int *src, double *tau, hipStream_t st);
+
+// adjoint launchers (rte_adjoint.hip)
+template <typename T, typename TA>
+struct AdjCombArgs {
+    GridDev<T> g;
+    int nchunk;
+    const TA *mom;        // [kNMom][K][K]
+    TA self_scale;        // dtheta dphi / dV
+    TA dV;
+    TA coefE[kMaxL + 1];
+    TA nsc[kMaxK];
+    T *X;                 // in: n (-1)^l xi_q; out: n (-1)^l xi_{q-1}
+    const T *Wq;          // n W_q
+    const T *part;
+    const TA *aw_prev;    // A W_{q-1} (unscaled) or nullptr when q == 0
+    const T *mut, *mus;
+    TA g_aniso;
+    const TA *cot;        // [nx*ny] cotangent of the image
+    TA *gt;               // [N] d/d mu_t accumulator (self block)
+    TA *gs;               // [N] d/d mu_s accumulator (scatter)
+    TA *lam0;             // [N][L+1] Lambda_0 m = 0 coefficients when q == 0, else nullptr
+};
+template <typename T, typename TA>
+hipError_t launch_adj_prop(int L, const PropArgs<T> &A, const T *X, const T *Wq, T *part, T *pairbuf,
+                           const T *ktab, hipStream_t st);
+template <typename T, typename TA>
+hipError_t launch_adj_combine(int L, const AdjCombArgs<T, TA> &A, hipStream_t st);
+template <typename T, typename TA>
+hipError_t launch_adj_init(int L, const GridDev<T> &g, const TA *cot, const Scales &sc, const TA *coefE, T *X,
+                           hipStream_t st);
+template <typename T, typename TA>
+hipError_t launch_make_wq(int L, const GridDev<T> &g, const Scales &sc, TA gan, const T *mus, const TA *aw_prev,
+                          T *Wq, hipStream_t st);
+template <typename T, typename TA>
+hipError_t launch_backproject(const GridDev<T> &g, int nchunk, int chunk_len, T thr2, const T *pairbuf, TA scale,
+                              TA *gt, hipStream_t st);
+template <typename TA>
+hipError_t launch_src_grad(int L, int nx, int ny, int nz, double dx, double dy, double dz, double xmin, double ymin,
+                           double gan, const BeamDev &B, const TA *lam0, const TA *cot, TA *ga, TA *gs, hipStream_t st);
 
 }  // namespace rte

@@ -17,34 +17,9 @@
 // structure, GPU package/src/kernels.cu:49-145).  Partial sums per source
 // chunk are combined in a fixed order by k_combine, so results do not depend
 // on how many solves share a launch.
-#include "rte_internal.hpp"
+#include "rte_device.hpp"
 
 namespace rte {
-
-// ---------------------------------------------------------------------------
-// small helpers
-// ---------------------------------------------------------------------------
-
-template <typename T>
-__device__ __forceinline__ T dev_rsqrt(T x);
-template <>
-__device__ __forceinline__ float dev_rsqrt<float>(float x) { return __frsqrt_rn(x); }
-template <>
-__device__ __forceinline__ double dev_rsqrt<double>(double x) { return 1.0 / sqrt(x); }
-
-template <typename T>
-__device__ __forceinline__ T dev_exp(T x);
-template <>
-__device__ __forceinline__ float dev_exp<float>(float x) { return expf(x); }
-template <>
-__device__ __forceinline__ double dev_exp<double>(double x) { return exp(x); }
-
-template <typename T>
-__device__ __forceinline__ T big_value();
-template <>
-__device__ __forceinline__ float big_value<float>() { return 1.0e30f; }
-template <>
-__device__ __forceinline__ double big_value<double>() { return 1.0e300; }
 
 template <typename T>
 __global__ void k_fill(T *__restrict__ p, size_t n, T v)
@@ -54,137 +29,22 @@ __global__ void k_fill(T *__restrict__ p, size_t n, T v)
     for (; i < n; i += stride) p[i] = v;
 }
 
-// ---------------------------------------------------------------------------
-// Siddon path integral between the centres of source s = (si,sj,sk) and
-// destination d = s + (ix,iy,iz):  tau = r * sum_v mu_t(v) dalpha(v).
-// Crossings along an axis with n steps sit at alpha = (j + 1/2)/n; ties go to
-// z, then y, then x; the destination contributes the half step 0.5/n of the
-// axis it was entered through (GPU package/src/kernels.cu:49-145).
-// ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ T march_tau(const __attribute__((address_space(1))) T *mut, int v, int ix, int iy, int iz,
-                                       int nx, int nxy, T r)
-{
-    const int n0 = abs(ix), n1 = abs(iy), n2 = abs(iz);
-    const T BIG = big_value<T>();
-    const T inc0 = n0 ? T(1) / T(n0) : BIG;
-    const T inc1 = n1 ? T(1) / T(n1) : BIG;
-    const T inc2 = n2 ? T(1) / T(n2) : BIG;
-    T a0 = T(0.5) * inc0, a1 = T(0.5) * inc1, a2 = T(0.5) * inc2;
-    const int st0 = ix > 0 ? 1 : -1, st1 = iy > 0 ? nx : -nx, st2 = iz > 0 ? nxy : -nxy;
-    const int nsteps = n0 + n1 + n2;
-    T prev = T(0), acc = T(0), last = T(0);
-    for (int t = 0; t < nsteps; ++t) {
-        const bool pz = (a2 <= a0) && (a2 <= a1);
-        const bool py = !pz && (a1 <= a0);
-        const T a = pz ? a2 : (py ? a1 : a0);
-        acc += mut[v] * (a - prev);
-        prev = a;
-        v += pz ? st2 : (py ? st1 : st0);
-        last = pz ? inc2 : (py ? inc1 : inc0);
-        a2 = pz ? a2 + inc2 : a2;
-        a1 = py ? a1 + inc1 : a1;
-        a0 = (!pz && !py) ? a0 + inc0 : a0;
-    }
-    acc += mut[v] * T(0.5) * last;
-    return acc * r;
-}
-
-// ---------------------------------------------------------------------------
-// One (destination, source) pair in the factored real-SH form, on scaled
-// coefficients (see ShConst):  p = sum_a Rt_a(u) w'_a ;  acc' += (kf p) Rt(u)
-// with Rt_l0 = Pt_l^0, Rt_lm = Pt_l^m C_m, Rt_l,-m = Pt_l^m S_m,
-// C_m + i S_m = (x + i y)^m.
-// ---------------------------------------------------------------------------
-template <typename T, int L>
-__device__ __forceinline__ void pair_accumulate(T (&acc)[(L + 1) * (L + 1)],
-                                                const T (&w)[(L + 1) * (L + 1)], T kf, T x, T y,
-                                                T z, const ShConst<T> &sh)
-{
-    T C[L + 1], S[L + 1];
-    C[0] = T(1);
-    S[0] = T(0);
-    if constexpr (L >= 1) {
-        C[1] = x;
-        S[1] = y;
-    }
-#pragma unroll
-    for (int m = 2; m <= L; ++m) {
-        C[m] = C[m - 1] * x - S[m - 1] * y;
-        S[m] = S[m - 1] * x + C[m - 1] * y;
-    }
-    T P[L + 1][L + 1];  // only l >= m used
-#pragma unroll
-    for (int m = 0; m <= L; ++m) {
-        P[m][m] = T(1);
-        if (m + 1 <= L) P[m + 1][m] = z;
-#pragma unroll
-        for (int l = m + 2; l <= L; ++l) P[l][m] = z * P[l - 1][m] - sh.gamma[l][m] * P[l - 2][m];
-    }
-    T p = w[0];
-#pragma unroll
-    for (int l = 1; l <= L; ++l) p += P[l][0] * w[l * l + l];
-#pragma unroll
-    for (int m = 1; m <= L; ++m) {
-        T A = w[m * m + m + m], B = w[m * m + m - m];
-#pragma unroll
-        for (int l = m + 1; l <= L; ++l) {
-            A += P[l][m] * w[l * l + l + m];
-            B += P[l][m] * w[l * l + l - m];
-        }
-        p += C[m] * A + S[m] * B;
-    }
-    const T t = kf * p;
-    acc[0] += t;
-#pragma unroll
-    for (int l = 1; l <= L; ++l) acc[l * l + l] += t * P[l][0];
-#pragma unroll
-    for (int m = 1; m <= L; ++m) {
-        const T u = t * C[m], vv = t * S[m];
-        acc[m * m + m + m] += u;
-        acc[m * m + m - m] += vv;
-#pragma unroll
-        for (int l = m + 1; l <= L; ++l) {
-            acc[l * l + l + m] += u * P[l][m];
-            acc[l * l + l - m] += vv * P[l][m];
-        }
-    }
-}
-
-// address-space views: wave-uniform reads through the scalar cache, per-lane
-// reads as global (not flat) loads
-template <typename T>
-using cptr = const __attribute__((address_space(4))) T *;
-template <typename T>
-using gptr = const __attribute__((address_space(1))) T *;
-template <typename T>
-__device__ __forceinline__ cptr<T> as_const(const T *p) { return (cptr<T>)(p); }
-template <typename T>
-__device__ __forceinline__ gptr<T> as_global(const T *p) { return (gptr<T>)(p); }
-
-// destination of a lane: brick blk, lane -> (di, dj, dk)
-template <typename T>
-__device__ __forceinline__ void lane_dest(const GridDev<T> &g, int blk, int lane, int &di, int &dj,
-                                          int &dk)
-{
-    const int bxi = blk % g.bx, byi = (blk / g.bx) % g.by, bzi = blk / (g.bx * g.by);
-    di = bxi * kBrick + (lane & 3);
-    dj = byi * kBrick + ((lane >> 2) & 3);
-    dk = bzi * kBrick + (lane >> 4);
-}
 
 // ---------------------------------------------------------------------------
 // Propagation: partial pair sums over one chunk of sources.
-//   grid = (nblk, nchunk, nsolve), block = 64
+//   grid = (nblk, nchunk, ngroups), block = 64; group z holds the NR solves
+//   sv[z*NR .. z*NR+NR-1], which share one medium (same path table and
+//   mu_t): the pair direction, its SH recurrences and kf are computed once
+//   and applied to every right-hand side.
 // PATH 0: kf from the per-medium table; PATH 1: marched per pair.
 // ---------------------------------------------------------------------------
-template <typename T, typename TA, int L, int PATH>
+template <typename T, typename TA, int L, int PATH, int NR>
 __global__ __launch_bounds__(kWave) void k_prop(PropArgs<T> A, const SolveDev<T, TA> *__restrict__ sv)
 {
     constexpr int K = (L + 1) * (L + 1);
     const GridDev<T> &g = A.g;
     const int lane = threadIdx.x, blk = blockIdx.x, chunk = blockIdx.y;
-    const SolveDev<T, TA> S = sv[blockIdx.z];
+    const SolveDev<T, TA> *__restrict__ grp = sv + (size_t)blockIdx.z * NR;
     int di, dj, dk;
     lane_dest(g, blk, lane, di, dj, dk);
     const bool valid = di < g.nx && dj < g.ny && dk < g.nz;
@@ -195,33 +55,58 @@ __global__ __launch_bounds__(kWave) void k_prop(PropArgs<T> A, const SolveDev<T,
     const int s1 = min(g.N, s0 + A.chunk_len);
     const int nxy = g.nx * g.ny;
 
-    T acc[K];
+    T acc[NR][K];
+    cptr<T> Wsrc[NR];
 #pragma unroll
-    for (int a = 0; a < K; ++a) acc[a] = T(0);
-
+    for (int r = 0; r < NR; ++r) {
+        Wsrc[r] = as_const<T>(grp[r].W);
+#pragma unroll
+        for (int a = 0; a < K; ++a) acc[r][a] = T(0);
+    }
+    const gptr<T> ktab = as_global<T>(grp[0].ktab);
+    const gptr<T> mut = as_global<T>(grp[0].mut);
+    // destination coordinates; source coordinates advance by dx along x
+    const T dxc = T(cdi) * g.dx, dyc = T(cdj) * g.dy, dzc = T(cdk) * g.dz;
     int si = s0 % g.nx, sj = (s0 / g.nx) % g.ny, sk = s0 / nxy;
-    const cptr<T> Wsrc = as_const<T>(S.W);
-    const gptr<T> ktab = as_global<T>(S.ktab);
-    for (int s = s0; s < s1; ++s) {
-        T w[K];
+    T kf_next = T(0);
+    if constexpr (PATH == 0) {
+        if (s0 < s1) kf_next = ktab[(size_t)s0 * g.npad + slot];
+    }
+    T w[K];
 #pragma unroll
-        for (int a = 0; a < K; ++a) w[a] = Wsrc[(size_t)s * K + a];
-        const int ix = cdi - si, iy = cdj - sj, iz = cdk - sk;
-        const T ddx = T(ix) * g.dx, ddy = T(iy) * g.dy, ddz = T(iz) * g.dz;
+    for (int a = 0; a < K; ++a) w[a] = (s0 < s1) ? Wsrc[0][(size_t)s0 * K + a] : T(0);
+    for (int s = s0; s < s1; ++s) {
+        const T ddx = dxc - T(si) * g.dx, ddy = dyc - T(sj) * g.dy, ddz = dzc - T(sk) * g.dz;
         T r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-        const bool self = (ix == 0) && (iy == 0) && (iz == 0);
+        const bool self = (cdi == si) && (cdj == sj) && (cdk == sk);
         T kf;
         if constexpr (PATH == 0) {
-            kf = ktab[(size_t)s * g.npad + slot];
+            kf = kf_next;
+            if (s + 1 < s1) kf_next = ktab[(size_t)(s + 1) * g.npad + slot];
         } else {
+            const int ix = cdi - si, iy = cdj - sj, iz = cdk - sk;
             const bool use = valid && !self && (r2 < A.thr2);
             const T r = sqrt(self ? T(1) : r2);
-            const T tau = use ? march_tau<T>(as_global<T>(S.mut), s, ix, iy, iz, g.nx, nxy, r) : T(0);
+            const T tau = use ? march_tau<T>(mut, s, ix, iy, iz, g.nx, nxy, r) : T(0);
             kf = use ? dev_exp<T>(-tau) / r2 : T(0);
         }
         r2 = self ? T(1) : r2;
         const T rinv = dev_rsqrt<T>(r2);
-        pair_accumulate<T, L>(acc, w, kf, ddx * rinv, ddy * rinv, ddz * rinv, A.sh);
+        PairGeom<T, L> G;
+        pair_geom<T, L>(G, ddx * rinv, ddy * rinv, ddz * rinv, A.sh);
+        // w holds right-hand side r of source s; the next one (r+1, or 0 of
+        // s+1) is requested before r is applied, hiding the scalar-load latency
+        const int snext = (s + 1 < s1) ? s + 1 : s;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            T wn[K];
+#pragma unroll
+            for (int a = 0; a < K; ++a)
+                wn[a] = (r + 1 < NR) ? Wsrc[r + 1][(size_t)s * K + a] : Wsrc[0][(size_t)snext * K + a];
+            pair_apply<T, L>(acc[r], w, kf, G);
+#pragma unroll
+            for (int a = 0; a < K; ++a) w[a] = wn[a];
+        }
         if (++si == g.nx) {
             si = 0;
             if (++sj == g.ny) {
@@ -232,9 +117,12 @@ __global__ __launch_bounds__(kWave) void k_prop(PropArgs<T> A, const SolveDev<T,
     }
     if (valid) {
         const int v = di + g.nx * (dj + g.ny * dk);
-        T *__restrict__ out = S.part + ((size_t)chunk * g.N + v) * K;
 #pragma unroll
-        for (int a = 0; a < K; ++a) out[a] = acc[a];
+        for (int r = 0; r < NR; ++r) {
+            T *__restrict__ out = grp[r].part + ((size_t)chunk * g.N + v) * K;
+#pragma unroll
+            for (int a = 0; a < K; ++a) out[a] = acc[r][a];
+        }
     }
 }
 
@@ -328,13 +216,17 @@ __global__ __launch_bounds__(kWave) void k_combine(CombArgs<T, TA> A, const Solv
         for (int a = 0; a < K; ++a) A.aw_out[(size_t)v * K + a] = acc[a];
         return;
     }
-    if (bottom) {
+    if (S.aw_store) {
+#pragma unroll
+        for (int a = 0; a < K; ++a) S.aw_store[(size_t)v * K + a] = acc[a];
+    }
+    if (bottom && S.mode != 3) {
         TA e = TA(0);
 #pragma unroll
         for (int l = 0; l <= L; ++l) e += A.coefE[l] * acc[l * l + l];
         S.dE[pix] = e;
     }
-    if (S.mode == 0) {
+    if (S.mode == 0 || S.mode == 3) {
         const TA ms = TA(S.mus[v]);
         TA gl = TA(1), bs = TA(0);
 #pragma unroll
@@ -349,7 +241,7 @@ __global__ __launch_bounds__(kWave) void k_combine(CombArgs<T, TA> A, const Solv
             }
             gl *= TA(S.g);
         }
-        if (bottom) S.bsum[pix] = bs;
+        if (bottom && S.mode == 0) S.bsum[pix] = bs;
     }
 }
 
@@ -572,15 +464,27 @@ hipError_t launch_fill(T *p, size_t n, T v, hipStream_t st)
 }
 
 template <typename T, typename TA>
-hipError_t launch_prop(int L, int path, const PropArgs<T> &A, const SolveDev<T, TA> *sv, int nsolve,
+hipError_t launch_prop(int L, int path, int nr, const PropArgs<T> &A, const SolveDev<T, TA> *sv, int ngroups,
                        hipStream_t st)
 {
-    dim3 grid(A.g.nblk, A.nchunk, nsolve), block(kWave);
+    dim3 grid(A.g.nblk, A.nchunk, ngroups), block(kWave);
+    // several right-hand sides per group only on the mixed-precision cached path
+    constexpr bool multi = sizeof(T) == 4 && sizeof(TA) == 8;
+    if (nr < 1 || nr > kMaxNR || (nr > 1 && (!multi || path != 0))) return hipErrorInvalidValue;
     RTE_DISPATCH_L(L, {
-        if (path == 0)
-            hipLaunchKernelGGL((k_prop<T, TA, LL, 0>), grid, block, 0, st, A, sv);
-        else
-            hipLaunchKernelGGL((k_prop<T, TA, LL, 1>), grid, block, 0, st, A, sv);
+        if (path == 1) {
+            hipLaunchKernelGGL((k_prop<T, TA, LL, 1, 1>), grid, block, 0, st, A, sv);
+        } else if constexpr (multi) {
+            switch (nr) {
+            case 1: hipLaunchKernelGGL((k_prop<T, TA, LL, 0, 1>), grid, block, 0, st, A, sv); break;
+            case 2: hipLaunchKernelGGL((k_prop<T, TA, LL, 0, 2>), grid, block, 0, st, A, sv); break;
+            case 3: hipLaunchKernelGGL((k_prop<T, TA, LL, 0, 3>), grid, block, 0, st, A, sv); break;
+            case 4: hipLaunchKernelGGL((k_prop<T, TA, LL, 0, 4>), grid, block, 0, st, A, sv); break;
+            default: hipLaunchKernelGGL((k_prop<T, TA, LL, 0, 5>), grid, block, 0, st, A, sv); break;
+            }
+        } else {
+            hipLaunchKernelGGL((k_prop<T, TA, LL, 0, 1>), grid, block, 0, st, A, sv);
+        }
     });
     return hipGetLastError();
 }
@@ -673,11 +577,11 @@ hipError_t launch_tau_list(const GridDev<float> &g, const float *mut, int np, co
 // explicit instantiations used by the host library
 template hipError_t launch_fill<float>(float *, size_t, float, hipStream_t);
 template hipError_t launch_fill<double>(double *, size_t, double, hipStream_t);
-template hipError_t launch_prop<float, double>(int, int, const PropArgs<float> &,
+template hipError_t launch_prop<float, double>(int, int, int, const PropArgs<float> &,
                                                const SolveDev<float, double> *, int, hipStream_t);
-template hipError_t launch_prop<double, double>(int, int, const PropArgs<double> &,
+template hipError_t launch_prop<double, double>(int, int, int, const PropArgs<double> &,
                                                 const SolveDev<double, double> *, int, hipStream_t);
-template hipError_t launch_prop<float, float>(int, int, const PropArgs<float> &,
+template hipError_t launch_prop<float, float>(int, int, int, const PropArgs<float> &,
                                               const SolveDev<float, float> *, int, hipStream_t);
 template hipError_t launch_ktab<float>(const GridDev<float> &, int, int, float, const float *, float *,
                                        hipStream_t);

@@ -57,6 +57,10 @@ class Stats(C.Structure):
                 ("gpu_ms", C.c_double), ("latency_ms", C.c_double)]
 
 
+class GradOpts(C.Structure):
+    _fields_ = [("checkpoint_stride", C.c_int)]
+
+
 class Prof(C.Structure):
     _fields_ = [("prop_launches", C.c_longlong), ("prop_ms", C.c_double), ("prop_pairs", C.c_double),
                 ("prop_flops", C.c_double), ("ktab_launches", C.c_longlong), ("ktab_ms", C.c_double),
@@ -74,6 +78,7 @@ SIGNATURES = {
     "rte_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
     "rte_ctx_destroy": (C.c_int, [_vp]),
     "rte_ctx_profile": (C.c_int, [_vp, C.c_int]),
+    "rte_ctx_set_group_size": (C.c_int, [_vp, C.c_int]),
     "rte_ctx_get_profile": (C.c_int, [_vp, C.POINTER(Prof)]),
     "rte_medium_create": (C.c_int, [_vp, C.POINTER(Grid), C.POINTER(Optics), C.POINTER(_vp)]),
     "rte_medium_destroy": (C.c_int, [_vp]),
@@ -83,6 +88,10 @@ SIGNATURES = {
                             C.POINTER(Params), _dp, C.POINTER(Stats)]),
     "rte_solve_media": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(Beam), C.POINTER(Params),
                                   _dp, C.POINTER(Stats)]),
+    "rte_exitance_vjp": (C.c_int, [_vp, _vp, C.POINTER(Beam), C.POINTER(Params), C.POINTER(GradOpts), _dp, _dp,
+                                   _dp, _dp, C.POINTER(Stats)]),
+    "rte_gradient": (C.c_int, [_vp, _vp, C.POINTER(Beam), C.POINTER(Params), C.POINTER(GradOpts), _dp, _dp, _dp,
+                               _dp, _dp, C.POINTER(Stats)]),
     "rte_propagate": (C.c_int, [_vp, C.POINTER(Grid), C.POINTER(Optics), C.c_int, C.c_double,
                                 C.c_int, C.c_int, _dp, _dp]),
     "rte_scatter": (C.c_int, [_vp, C.POINTER(Grid), C.POINTER(Optics), C.c_int, _dp]),
@@ -181,6 +190,9 @@ class Context:
     def __exit__(self, *a):
         self.close()
 
+    def set_group_size(self, max_rhs):
+        _check(lib().rte_ctx_set_group_size(self.h, max_rhs))
+
     def profile(self, on=True):
         _check(lib().rte_ctx_profile(self.h, 1 if on else 0))
 
@@ -211,6 +223,32 @@ class Context:
         finally:
             m.close()
         return img[0], st[0]
+
+    # ---- derivatives -----------------------------------------------------
+    def exitance_vjp(self, medium, beam, params, cotangent, checkpoint_stride=1):
+        nx, ny, _ = medium.shape
+        cot = _f64(np.ravel(cotangent))
+        img = np.zeros(nx * ny)
+        ga = np.zeros(medium.N)
+        gs = np.zeros(medium.N)
+        st = Stats()
+        _check(lib().rte_exitance_vjp(self.h, medium.h, C.byref(Beam(*beam)), C.byref(params),
+                                      C.byref(GradOpts(checkpoint_stride)), _p(cot), _p(img), _p(ga), _p(gs),
+                                      C.byref(st)))
+        return img.reshape(ny, nx), ga, gs, dict(terms=st.terms, gpu_ms=st.gpu_ms)
+
+    def gradient(self, medium, beam, params, target, checkpoint_stride=1):
+        nx, ny, _ = medium.shape
+        tgt = _f64(np.ravel(target))
+        img = np.zeros(nx * ny)
+        ga = np.zeros(medium.N)
+        gs = np.zeros(medium.N)
+        loss = C.c_double(0.0)
+        st = Stats()
+        _check(lib().rte_gradient(self.h, medium.h, C.byref(Beam(*beam)), C.byref(params),
+                                  C.byref(GradOpts(checkpoint_stride)), _p(tgt), C.byref(loss), _p(img), _p(ga),
+                                  _p(gs), C.byref(st)))
+        return loss.value, img.reshape(ny, nx), ga, gs, dict(terms=st.terms, gpu_ms=st.gpu_ms)
 
     # ---- operators -------------------------------------------------------
     def propagate(self, shape, bounds, mu_a, mu_s, g, W, L, prop_thresh=0.0, precision=PREC_FP64,

@@ -129,6 +129,9 @@ typedef struct rte_medium rte_medium;
 int rte_ctx_create(int device, rte_ctx **out);
 /* also destroys the media still attached to the context (their handles become invalid) */
 int rte_ctx_destroy(rte_ctx *ctx);
+/* at most max_rhs (1..5, default 5) beams on one medium share a propagation
+ * launch on the mixed-precision cached path; the result bits do not depend on it */
+int rte_ctx_set_group_size(rte_ctx *ctx, int max_rhs);
 /* on != 0: time every launch from now on (resets the counters); 0: stop */
 int rte_ctx_profile(rte_ctx *ctx, int on);
 int rte_ctx_get_profile(rte_ctx *ctx, rte_prof *out);
@@ -159,6 +162,30 @@ int rte_solve(rte_ctx *ctx, const rte_grid *grid, const rte_optics *optics,
  * propagations).  exitance is nb*nx*ny, stats nb entries (may be NULL). */
 int rte_solve_media(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_beam *beams,
                     const rte_params *params, double *exitance, rte_stats *stats);
+
+/* ---- derivatives ----------------------------------------------------------
+ * Reverse mode through the Neumann series as it ran (same term count): one
+ * adjoint series with the transpose of A (reciprocity) and one back-projection
+ * of the pair weights along the rays give the derivative with respect to
+ * every voxel's mu_a and mu_s, including the source and ballistic terms.  The
+ * reference instead runs one extra forward solve per parameter with the source
+ * -c w restricted to the parameter's voxels (FisherInfo_IO/src/compute_grad_rte.cu:
+ * 273-385, GPU package/src/inverse_rte.cu:290-433).  Needs RTE_PATH_CACHED and
+ * RTE_PREC_FP64 or RTE_PREC_MIXED. */
+typedef struct {
+    int checkpoint_stride; /* keep A W_q of every stride-th term of the forward pass and
+                              recompute the others in the backward pass (>= 1; 1 keeps all) */
+} rte_grad_opts;
+
+/* dmu_a[v] = sum_p cotangent[p] dE_p/dmu_a(v), same for dmu_s (N each, may be NULL);
+ * exitance (nx*ny, may be NULL) receives E */
+int rte_exitance_vjp(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte_params *params,
+                     const rte_grad_opts *opts, const double *cotangent, double *exitance,
+                     double *dmu_a, double *dmu_s, rte_stats *stats);
+/* loss = sum_p (E_p - target_p)^2 and its gradient (FisherInfo_IO/src/compute_grad_rte.cu:315) */
+int rte_gradient(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte_params *params,
+                 const rte_grad_opts *opts, const double *target, double *loss, double *exitance,
+                 double *dmu_a, double *dmu_s, rte_stats *stats);
 
 /* ---- operators (for operator-level parity) ------------------------------ */
 /* A: propagation incl. self-voxel block and dV (CPU package/src/Kernels_selfsub.c:450-538,

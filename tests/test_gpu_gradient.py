@@ -1,0 +1,110 @@
+"""Reverse-mode derivatives of the exitance (rte_exitance_vjp / rte_gradient)
+against central finite differences of the forward solve with the Neumann
+term count held fixed -- on the GPU in FP64 and on the FP64 oracle -- plus
+checkpointing and precision invariants.
+
+Tolerance: |adjoint - FD| <= 2e-6 * max|FD| + 1e-5 |FD| (FP64, step 1e-4 relative)."""
+import numpy as np
+import pytest
+
+from doi_amd import rte
+from oracle import oracle as O
+from tests.helpers import bounds_for, hetero_maps
+
+pytestmark = pytest.mark.gpu
+
+SHAPE, L, G = (6, 6, 5), 3, 0.8
+BEAM = (0.05, -0.05, 0.1, 1.0)
+
+
+def _setup(seed=11):
+    bounds = bounds_for(SHAPE)
+    ma, ms = hetero_maps(SHAPE, seed=seed, sigma=0.3)
+    cot = np.random.default_rng(seed).standard_normal(SHAPE[0] * SHAPE[1])
+    return bounds, ma, ms, cot
+
+
+def _fd(fun, ma, ms, which, v, h_rel=1e-4):
+    a, s = ma.copy(), ms.copy()
+    base = (a if which == "a" else s)[v]
+    h = h_rel * base
+    (a if which == "a" else s)[v] = base + h
+    fp = fun(a, s)
+    (a if which == "a" else s)[v] = base - h
+    fm = fun(a, s)
+    return (fp - fm) / (2 * h)
+
+
+def _close(adj, fd, scale):
+    return abs(adj - fd) <= 2e-6 * scale + 1e-5 * abs(fd)
+
+
+def test_vjp_matches_finite_differences_gpu_and_oracle(ctx):
+    bounds, ma, ms, cot = _setup()
+    p = rte.make_params(L, precision=rte.PREC_FP64, stop_rule=rte.STOP_DELTA, tol=1e-8, max_terms=40)
+    med = ctx.medium(SHAPE, bounds, ma, ms, G)
+    img, ga, gs, st = ctx.exitance_vjp(med, BEAM, p, cot)
+    n = st["terms"]
+    med.close()
+    pf = rte.make_params(L, precision=rte.PREC_FP64, stop_rule=rte.STOP_DELTA, tol=0.0, max_terms=n)
+
+    def gpu_fun(a, s):
+        e, stt = ctx.solve(SHAPE, bounds, a, s, G, BEAM, pf)
+        assert stt["terms"] == n
+        return float(np.dot(cot, e.ravel()))
+
+    P = O.Problem(SHAPE, bounds, ma, ms, G)
+
+    def orc_fun(a, s):
+        r = O.Problem(SHAPE, bounds, a, s, G).solve(BEAM, L, stop_rule=O.STOP_DELTA, tol=0.0, max_terms=n)
+        return float(np.dot(cot, r["exitance"].ravel()))
+
+    N = int(np.prod(SHAPE))
+    rng = np.random.default_rng(3)
+    # a beam column voxel, a bottom voxel, and random ones
+    vox = [2 + 6 * 2 + 36 * 1, 3 + 6 * 3 + 36 * 4] + list(rng.choice(N, 8, replace=False))
+    scale = max(np.max(np.abs(ga)), np.max(np.abs(gs)))
+    for v in vox:
+        for which, adj in (("a", ga[v]), ("s", gs[v])):
+            fd_g = _fd(gpu_fun, ma, ms, which, v)
+            assert _close(adj, fd_g, scale), (v, which, adj, fd_g)
+    for v in vox[:4]:
+        for which, adj in (("a", ga[v]), ("s", gs[v])):
+            fd_o = _fd(orc_fun, ma, ms, which, v)
+            assert _close(adj, fd_o, scale), (v, which, adj, fd_o)
+    # the image of the vjp call is the forward solve's
+    ref = P.solve(BEAM, L, stop_rule=O.STOP_DELTA, tol=1e-8, max_terms=40)
+    assert np.max(np.abs(img - ref["exitance"])) < 1e-9 * np.max(np.abs(ref["exitance"]))
+
+
+def test_checkpoint_stride_is_exact(ctx):
+    """Recomputing the terms between checkpoints reproduces them exactly; the
+    back-projection adds with atomics, so the gradients agree to rounding."""
+    bounds, ma, ms, cot = _setup(seed=5)
+    med = ctx.medium(SHAPE, bounds, ma, ms, G)
+    p = rte.make_params(L, precision=rte.PREC_FP64, stop_rule=rte.STOP_LAMBDA)
+    _, ga1, gs1, st1 = ctx.exitance_vjp(med, BEAM, p, cot, checkpoint_stride=1)
+    for stride in (2, 3, 7):
+        _, ga, gs, st = ctx.exitance_vjp(med, BEAM, p, cot, checkpoint_stride=stride)
+        assert st["terms"] == st1["terms"]
+        sc = max(np.max(np.abs(ga1)), np.max(np.abs(gs1)))
+        assert np.max(np.abs(ga - ga1)) < 1e-12 * sc and np.max(np.abs(gs - gs1)) < 1e-12 * sc
+    med.close()
+
+
+def test_loss_gradient_and_mixed_precision(ctx):
+    bounds, ma, ms, _ = _setup(seed=8)
+    ma2, ms2 = hetero_maps(SHAPE, seed=99, sigma=0.3)
+    med = ctx.medium(SHAPE, bounds, ma, ms, G)
+    p64 = rte.make_params(L, precision=rte.PREC_FP64, stop_rule=rte.STOP_DELTA, tol=1e-7, max_terms=30)
+    target, _ = ctx.solve(SHAPE, bounds, ma2, ms2, G, BEAM, p64)
+    loss, img, ga, gs, st = ctx.gradient(med, BEAM, p64, target)
+    assert abs(loss - np.sum((img - target) ** 2)) < 1e-12 * max(loss, 1e-30)
+    _, ga2, gs2, _ = ctx.exitance_vjp(med, BEAM, p64, 2 * (img - target))
+    assert np.allclose(ga, ga2, rtol=1e-12, atol=0) and np.allclose(gs, gs2, rtol=1e-12, atol=0)
+    pm = rte.make_params(L, precision=rte.PREC_MIXED, stop_rule=rte.STOP_DELTA, tol=1e-7, max_terms=30)
+    lm, _, gam, gsm, stm = ctx.gradient(med, BEAM, pm, target)
+    assert stm["terms"] == st["terms"]
+    sc = max(np.max(np.abs(ga)), np.max(np.abs(gs)))
+    assert np.max(np.abs(gam - ga)) < 1e-4 * sc and np.max(np.abs(gsm - gs)) < 1e-4 * sc
+    med.close()

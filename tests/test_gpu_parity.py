@@ -146,3 +146,20 @@ def test_batch_equals_alone(ctx):
     assert np.array_equal(batch[3], other)
     for m in mm:
         m.close()
+
+
+def test_beams_sharing_a_medium_equal_alone(ctx):
+    """Seven beams on one medium (propagated as groups of 5 and 2 right-hand
+    sides) give the same bits as each beam solved alone."""
+    shape, L = (12, 12, 12), 5
+    bounds = bounds_for(shape)
+    m = ctx.medium(shape, bounds, *hetero_maps(shape, seed=31), 0.9)
+    xs = (np.arange(12) + 0.5) * 0.1 - 0.6
+    beams = [(xs[i], xs[(3 * i + 2) % 12], 0.1, 1.0) for i in range(7)]
+    p = rte.make_params(L, precision=rte.PREC_MIXED, stop_rule=rte.STOP_DELTA, tol=1e-6, max_terms=20)
+    batch, st = ctx.solve_media([m] * 7, beams, p)
+    for i, b in enumerate(beams):
+        alone, st1 = ctx.solve_media([m], [b], p)
+        assert st1[0]["terms"] == st[i]["terms"]
+        assert np.array_equal(alone[0], batch[i])
+    m.close()
