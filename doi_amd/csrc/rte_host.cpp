@@ -1005,252 +1005,275 @@ extern "C" long long rte_solve_footprint(const rte_grid *grid, const rte_params 
 }
 
 // ---------------------------------------------------------------------------
-// reverse mode: vector-Jacobian product of the exitance (rte_adjoint.hip)
+// reverse mode: vector-Jacobian products of the exitance (rte_adjoint.hip)
 // ---------------------------------------------------------------------------
+// One forward series (A W_q kept at the checkpoints), then any number of
+// backward passes, one per cotangent of the image.
 template <typename T, typename TA>
-static int vjp_impl(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte_params *p, int stride,
-                    const double *cot_in, const double *target, double *loss, double *exitance, double *dmu_a,
-                    double *dmu_s, rte_stats *stats)
-{
-    const rte_grid &grid = m->grid;
-    const int L = p->L, K = (L + 1) * (L + 1);
-    const GridDev<T> g = make_grid<T>(grid);
-    const int N = g.N, P = grid.nx * grid.ny;
-    const double dx = (grid.xmax - grid.xmin) / grid.nx, dy = (grid.ymax - grid.ymin) / grid.ny,
-                 dz = (grid.zmax - grid.zmin) / grid.nz, dV = dx * dy * dz;
-    const double thr = p->prop_thresh > 0 ? p->prop_thresh : -1.0;
-    const int max_terms = p->max_terms > 0 ? p->max_terms : 1000;
-    hipStream_t st = ctx->stream;
-    int rc;
-    MomentSet *mom = nullptr;
-    if ((rc = get_moments(ctx, L, dx, dy, dz, &mom))) return rc;
-    HIP_TRY(hipEventRecord(ctx->ev0, st));
-    if ((rc = ensure_ktab<T>(m, thr))) return rc;
-    const int nch0 = pick_chunks(g.nblk, 1, N);
-    const int clen = (N + nch0 - 1) / nch0;
-    const int nch = (N + clen - 1) / clen;
-    const size_t F = (size_t)N * K;
-
+struct AdjointRun {
+    rte_ctx *ctx = nullptr;
+    rte_medium *m = nullptr;
+    const rte_params *p = nullptr;
+    int stride = 1, L = 0, K = 0, N = 0, P = 0, n = 0, nch = 0, clen = 0;
+    double dx = 0, dy = 0, dz = 0, dV = 0;
+    size_t F = 0;
+    GridDev<T> g{};
+    Scales sc{};
+    BeamDev bd{};
+    PropArgs<T> pa{};
+    CombArgs<T, TA> ca{};
+    double coefE[kMaxL + 1] = {};
+    HostSolve h{};
+    SolveDev<T, TA> sv{};
+    std::vector<TA> E;
     DevBuf dW, dS0, dPart, dPix, dRed, dSv, dBeam, dCot, dCoefE, dX, dWq, dPair, dGt, dGs, dGa2, dGs2, dLam0;
-    HIP_TRY(dW.alloc(F * sizeof(T)));
-    HIP_TRY(dS0.alloc(F * sizeof(T)));
-    HIP_TRY(dPart.alloc((size_t)nch * F * sizeof(T)));
-    HIP_TRY(dPix.alloc((size_t)3 * P * sizeof(TA)));
-    HIP_TRY(dRed.alloc(2 * sizeof(TA)));
-    HIP_TRY(dSv.alloc(sizeof(SolveDev<T, TA>)));
-    HIP_TRY(dBeam.alloc(sizeof(BeamDev)));
+    std::vector<std::unique_ptr<DevBuf>> ckpt, seg;
+    bool backward_ready = false;
 
-    // ---- forward series, keeping A W_q at the checkpoints ----
-    const double half = (beam->diam + 1.0e-16) / 2.0;
-    int cols = 0;
-    for (int j = 0; j < grid.ny; ++j)
-        for (int i = 0; i < grid.nx; ++i) {
-            const double xc = (double)i * dx + grid.xmin + dx / 2.0, yc = (double)j * dy + grid.ymin + dy / 2.0;
-            if (std::sqrt((xc - beam->pos_x) * (xc - beam->pos_x) + (yc - beam->pos_y) * (yc - beam->pos_y)) <= half)
-                ++cols;
-        }
-    if (cols == 0) return fail(RTE_E_ARG, "beam hits no voxel column");
-    const BeamDev bd{beam->pos_x, beam->pos_y, half, beam->mag / cols, m->mua_d.as<double>(), m->mus_d.as<double>()};
-    SolveDev<T, TA> sv;
-    std::memset(&sv, 0, sizeof sv);
-    sv.W = dW.as<T>();
-    sv.part = dPart.as<T>();
-    sv.ktab = sizeof(T) == 4 ? (const T *)m->ktab_f.p : (const T *)m->ktab_d.p;
-    sv.mut = sizeof(T) == 4 ? (const T *)m->mut_f.p : (const T *)m->mut_d.p;
-    sv.mus = sizeof(T) == 4 ? (const T *)m->mus_f.p : (const T *)m->mus_d.p;
-    sv.g = T(m->g);
-    sv.dE = dPix.as<TA>();
-    sv.bsum = sv.dE + P;
-    sv.E = sv.dE + 2 * P;
-    sv.red = dRed.as<TA>();
-    sv.mode = 1;
-    SolveDev<T, TA> *svd = dSv.as<SolveDev<T, TA>>();
-    const Scales sc = make_scales(L);
-    HIP_TRY(hipMemcpyAsync(svd, &sv, sizeof sv, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(dBeam.p, &bd, sizeof bd, hipMemcpyHostToDevice, st));
-    HIP_TRY(launch_fill<T>(dW.as<T>(), F, T(0), st));
-    HIP_TRY(launch_source<T, TA>(L, g, dx, dy, dz, grid.xmin, grid.ymin, sc, dBeam.as<BeamDev>(), svd, 1, st));
-    HIP_TRY(launch_bottom<T, TA>(P, svd, 1, st));
-    HIP_TRY(hipMemcpyAsync(dS0.p, dW.p, F * sizeof(T), hipMemcpyDeviceToDevice, st));
-    TA red[2];
-    HIP_TRY(hipMemcpyAsync(red, dRed.p, sizeof red, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    HostSolve h{m, 0, 0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0};
-    h.ball = (double)red[0];
-    h.T = h.ball;
-    if (p->stop_rule == RTE_STOP_LAMBDA) {
-        h.n = 1;
-        h.sumOut = (double)red[1];
-        h.state = (max_terms <= 1) ? 2 : 0;
-    }
+    int forward(rte_ctx *c, rte_medium *med, const rte_beam *beam, const rte_params *prm, int strd)
+    {
+        ctx = c;
+        m = med;
+        p = prm;
+        stride = strd;
+        const rte_grid &grid = m->grid;
+        L = p->L;
+        K = (L + 1) * (L + 1);
+        g = make_grid<T>(grid);
+        N = g.N;
+        P = grid.nx * grid.ny;
+        dx = (grid.xmax - grid.xmin) / grid.nx;
+        dy = (grid.ymax - grid.ymin) / grid.ny;
+        dz = (grid.zmax - grid.zmin) / grid.nz;
+        dV = dx * dy * dz;
+        const double thr = p->prop_thresh > 0 ? p->prop_thresh : -1.0;
+        const int max_terms = p->max_terms > 0 ? p->max_terms : 1000;
+        hipStream_t st = ctx->stream;
+        int rc;
+        MomentSet *mom = nullptr;
+        if ((rc = get_moments(ctx, L, dx, dy, dz, &mom))) return rc;
+        HIP_TRY(hipEventRecord(ctx->ev0, st));
+        if ((rc = ensure_ktab<T>(m, thr))) return rc;
+        const int nch0 = pick_chunks(g.nblk, 1, N);
+        clen = (N + nch0 - 1) / nch0;
+        nch = (N + clen - 1) / clen;
+        F = (size_t)N * K;
+        HIP_TRY(dW.alloc(F * sizeof(T)));
+        HIP_TRY(dS0.alloc(F * sizeof(T)));
+        HIP_TRY(dPart.alloc((size_t)nch * F * sizeof(T)));
+        HIP_TRY(dPix.alloc((size_t)3 * P * sizeof(TA)));
+        HIP_TRY(dRed.alloc(2 * sizeof(TA)));
+        HIP_TRY(dSv.alloc(sizeof(SolveDev<T, TA>)));
+        HIP_TRY(dBeam.alloc(sizeof(BeamDev)));
 
-    PropArgs<T> pa;
-    pa.g = g;
-    pa.sh = make_sh_const<T>();
-    pa.nchunk = nch;
-    pa.chunk_len = clen;
-    pa.thr2 = thr > 0 ? T(thr * thr) : T(sizeof(T) == 4 ? 1e30 : 1e300);
-    CombArgs<T, TA> ca;
-    std::memset(&ca, 0, sizeof ca);
-    ca.g = g;
-    ca.nchunk = nch;
-    ca.mom = sizeof(TA) == 4 ? (const TA *)mom->mf.p : (const TA *)mom->md.p;
-    ca.self_scale = TA((M_PI / kAngRes) * (2.0 * M_PI / kAngRes) / dV);
-    ca.dV = TA(dV);
-    double Q[kMaxL + 1], coefE[kMaxL + 1];
-    quad_weights(L, p->exit_quad, Q);
-    for (int l = 0; l <= L; ++l) {
-        coefE[l] = 2.0 * M_PI * dx * dy / std::sqrt(4.0 * M_PI) * std::sqrt(2.0 * l + 1.0) * Q[l];
-        ca.coefE[l] = TA(coefE[l]);
-    }
-    for (int a = 0; a < K; ++a) ca.wsum[a] = TA(stop_weight(a));
-    for (int a = 0; a < kMaxK; ++a) ca.nsc[a] = TA(sc.n[a]);
-
-    // checkpoints: A W_q is kept when (q + 1) % stride == 0 (all of them for stride 1)
-    std::vector<std::unique_ptr<DevBuf>> ckpt;
-    auto ckpt_slot = [&](int q) -> TA * {
-        if ((q + 1) % stride != 0) return nullptr;
-        const size_t idx = (size_t)q;
-        while (ckpt.size() <= idx) ckpt.emplace_back(nullptr);
-        if (!ckpt[idx]) {
-            ckpt[idx].reset(new DevBuf());
-            if (ckpt[idx]->alloc(F * sizeof(TA)) != hipSuccess) return nullptr;
-        }
-        return ckpt[idx]->as<TA>();
-    };
-    int q = 0;
-    while (h.state != 3) {
-        sv.mode = h.state;
-        sv.aw_store = ckpt_slot(q);
-        if ((q + 1) % stride == 0 && !sv.aw_store) return fail(RTE_E_NOMEM, "checkpoint allocation failed");
+        const double half = (beam->diam + 1.0e-16) / 2.0;
+        int cols = 0;
+        for (int j = 0; j < grid.ny; ++j)
+            for (int i = 0; i < grid.nx; ++i) {
+                const double xc = (double)i * dx + grid.xmin + dx / 2.0, yc = (double)j * dy + grid.ymin + dy / 2.0;
+                if (std::sqrt((xc - beam->pos_x) * (xc - beam->pos_x) + (yc - beam->pos_y) * (yc - beam->pos_y)) <=
+                    half)
+                    ++cols;
+            }
+        if (cols == 0) return fail(RTE_E_ARG, "beam hits no voxel column");
+        bd = BeamDev{beam->pos_x, beam->pos_y, half, beam->mag / cols, m->mua_d.as<double>(), m->mus_d.as<double>()};
+        std::memset(&sv, 0, sizeof sv);
+        sv.W = dW.as<T>();
+        sv.part = dPart.as<T>();
+        sv.ktab = sizeof(T) == 4 ? (const T *)m->ktab_f.p : (const T *)m->ktab_d.p;
+        sv.mut = sizeof(T) == 4 ? (const T *)m->mut_f.p : (const T *)m->mut_d.p;
+        sv.mus = sizeof(T) == 4 ? (const T *)m->mus_f.p : (const T *)m->mus_d.p;
+        sv.g = T(m->g);
+        sv.dE = dPix.as<TA>();
+        sv.bsum = sv.dE + P;
+        sv.E = sv.dE + 2 * P;
+        sv.red = dRed.as<TA>();
+        sv.mode = 1;
+        SolveDev<T, TA> *svd = dSv.as<SolveDev<T, TA>>();
+        sc = make_scales(L);
         HIP_TRY(hipMemcpyAsync(svd, &sv, sizeof sv, hipMemcpyHostToDevice, st));
-        HIP_TRY(launch_prop<T, TA>(L, RTE_PATH_CACHED, 1, pa, svd, 1, st));
-        HIP_TRY(launch_combine<T, TA>(L, ca, svd, 1, st));
+        HIP_TRY(hipMemcpyAsync(dBeam.p, &bd, sizeof bd, hipMemcpyHostToDevice, st));
+        HIP_TRY(launch_fill<T>(dW.as<T>(), F, T(0), st));
+        HIP_TRY(launch_source<T, TA>(L, g, dx, dy, dz, grid.xmin, grid.ymin, sc, dBeam.as<BeamDev>(), svd, 1, st));
         HIP_TRY(launch_bottom<T, TA>(P, svd, 1, st));
+        HIP_TRY(hipMemcpyAsync(dS0.p, dW.p, F * sizeof(T), hipMemcpyDeviceToDevice, st));
+        TA red[2];
         HIP_TRY(hipMemcpyAsync(red, dRed.p, sizeof red, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
-        advance_series(h, (double)red[0], (double)red[1], p->stop_rule, p->tol, max_terms);
-        ++q;
-    }
-    const int n = q;   // W_0 .. W_{n-1} entered the exitance
-
-    // ---- image, loss and cotangent ----
-    std::vector<TA> E(P);
-    HIP_TRY(hipMemcpy(E.data(), sv.E, sizeof(TA) * P, hipMemcpyDeviceToHost));
-    std::vector<TA> cot(P);
-    double lsum = 0.0;
-    for (int i = 0; i < P; ++i) {
-        if (exitance) exitance[i] = (double)E[i];
-        if (target) {
-            const double d = (double)E[i] - target[i];
-            lsum += d * d;
-            cot[i] = TA(2.0 * d);
-        } else {
-            cot[i] = TA(cot_in[i]);
+        h = HostSolve{m, 0, 0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0};
+        h.ball = (double)red[0];
+        h.T = h.ball;
+        if (p->stop_rule == RTE_STOP_LAMBDA) {
+            h.n = 1;
+            h.sumOut = (double)red[1];
+            h.state = (max_terms <= 1) ? 2 : 0;
         }
+        pa.g = g;
+        pa.sh = make_sh_const<T>();
+        pa.nchunk = nch;
+        pa.chunk_len = clen;
+        pa.thr2 = thr > 0 ? T(thr * thr) : T(sizeof(T) == 4 ? 1e30 : 1e300);
+        std::memset(&ca, 0, sizeof ca);
+        ca.g = g;
+        ca.nchunk = nch;
+        ca.mom = sizeof(TA) == 4 ? (const TA *)mom->mf.p : (const TA *)mom->md.p;
+        ca.self_scale = TA((M_PI / kAngRes) * (2.0 * M_PI / kAngRes) / dV);
+        ca.dV = TA(dV);
+        double Q[kMaxL + 1];
+        quad_weights(L, p->exit_quad, Q);
+        for (int l = 0; l <= L; ++l) {
+            coefE[l] = 2.0 * M_PI * dx * dy / std::sqrt(4.0 * M_PI) * std::sqrt(2.0 * l + 1.0) * Q[l];
+            ca.coefE[l] = TA(coefE[l]);
+        }
+        for (int a = 0; a < K; ++a) ca.wsum[a] = TA(stop_weight(a));
+        for (int a = 0; a < kMaxK; ++a) ca.nsc[a] = TA(sc.n[a]);
+
+        // A W_q is kept when (q + 1) % stride == 0 (every term for stride 1)
+        int q = 0;
+        while (h.state != 3) {
+            sv.mode = h.state;
+            sv.aw_store = nullptr;
+            if ((q + 1) % stride == 0) {
+                while (ckpt.size() <= (size_t)q) ckpt.emplace_back(nullptr);
+                ckpt[q].reset(new DevBuf());
+                HIP_TRY(ckpt[q]->alloc(F * sizeof(TA)));
+                sv.aw_store = ckpt[q]->as<TA>();
+            }
+            HIP_TRY(hipMemcpyAsync(svd, &sv, sizeof sv, hipMemcpyHostToDevice, st));
+            HIP_TRY(launch_prop<T, TA>(L, RTE_PATH_CACHED, 1, pa, svd, 1, st));
+            HIP_TRY(launch_combine<T, TA>(L, ca, svd, 1, st));
+            HIP_TRY(launch_bottom<T, TA>(P, svd, 1, st));
+            HIP_TRY(hipMemcpyAsync(red, dRed.p, sizeof red, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            advance_series(h, (double)red[0], (double)red[1], p->stop_rule, p->tol, max_terms);
+            ++q;
+        }
+        n = q;  // W_0 .. W_{n-1} entered the exitance
+        E.resize(P);
+        HIP_TRY(hipMemcpy(E.data(), sv.E, sizeof(TA) * P, hipMemcpyDeviceToHost));
+        return RTE_OK;
     }
-    if (loss) *loss = lsum;
-    HIP_TRY(dCot.alloc(sizeof(TA) * P));
-    HIP_TRY(dCoefE.alloc(sizeof(TA) * (kMaxL + 1)));
-    HIP_TRY(hipMemcpyAsync(dCot.p, cot.data(), sizeof(TA) * P, hipMemcpyHostToDevice, st));
-    TA cE[kMaxL + 1];
-    for (int l = 0; l <= kMaxL; ++l) cE[l] = l <= L ? TA(coefE[l]) : TA(0);
-    HIP_TRY(hipMemcpyAsync(dCoefE.p, cE, sizeof cE, hipMemcpyHostToDevice, st));
 
-    // ---- adjoint series ----
-    HIP_TRY(dX.alloc(F * sizeof(T)));
-    HIP_TRY(dWq.alloc(F * sizeof(T)));
-    HIP_TRY(dPair.alloc((size_t)N * g.npad * sizeof(T)));
-    HIP_TRY(dGt.alloc(N * sizeof(TA)));
-    HIP_TRY(dGs.alloc(N * sizeof(TA)));
-    HIP_TRY(dGa2.alloc(N * sizeof(TA)));
-    HIP_TRY(dGs2.alloc(N * sizeof(TA)));
-    HIP_TRY(dLam0.alloc((size_t)N * (L + 1) * sizeof(TA)));
-    HIP_TRY(launch_fill<T>(dPair.as<T>(), (size_t)N * g.npad, T(0), st));
-    HIP_TRY(launch_fill<TA>(dGt.as<TA>(), N, TA(0), st));
-    HIP_TRY(launch_fill<TA>(dGs.as<TA>(), N, TA(0), st));
-    HIP_TRY(launch_fill<TA>(dGa2.as<TA>(), N, TA(0), st));
-    HIP_TRY(launch_fill<TA>(dGs2.as<TA>(), N, TA(0), st));
-    HIP_TRY(launch_adj_init<T, TA>(L, g, dCot.as<TA>(), sc, dCoefE.as<TA>(), dX.as<T>(), st));
-
-    // the anisotropy exactly as the forward scatter used it (compute precision)
-    const TA gan = TA(T(m->g));
-    AdjCombArgs<T, TA> aa;
-    std::memset(&aa, 0, sizeof aa);
-    aa.g = g;
-    aa.nchunk = nch;
-    aa.mom = ca.mom;
-    aa.self_scale = ca.self_scale;
-    aa.dV = ca.dV;
-    for (int l = 0; l <= kMaxL; ++l) aa.coefE[l] = ca.coefE[l];
-    for (int a = 0; a < kMaxK; ++a) aa.nsc[a] = ca.nsc[a];
-    aa.X = dX.as<T>();
-    aa.Wq = dWq.as<T>();
-    aa.part = dPart.as<T>();
-    aa.mut = sv.mut;
-    aa.mus = sv.mus;
-    aa.g_aniso = gan;
-    aa.cot = dCot.as<TA>();
-    aa.gt = dGt.as<TA>();
-    aa.gs = dGs.as<TA>();
-
-    // segment buffer for recomputed A W_q between checkpoints
-    std::vector<std::unique_ptr<DevBuf>> seg;
-    for (int i = 0; i + 1 < stride; ++i) {
-        seg.emplace_back(new DevBuf());
-        HIP_TRY(seg.back()->alloc(F * sizeof(TA)));
+    int prepare_backward()
+    {
+        if (backward_ready) return RTE_OK;
+        HIP_TRY(dCot.alloc(sizeof(TA) * P));
+        HIP_TRY(dCoefE.alloc(sizeof(TA) * (kMaxL + 1)));
+        TA cE[kMaxL + 1];
+        for (int l = 0; l <= kMaxL; ++l) cE[l] = l <= L ? TA(coefE[l]) : TA(0);
+        HIP_TRY(hipMemcpy(dCoefE.p, cE, sizeof cE, hipMemcpyHostToDevice));
+        HIP_TRY(dX.alloc(F * sizeof(T)));
+        HIP_TRY(dWq.alloc(F * sizeof(T)));
+        HIP_TRY(dPair.alloc((size_t)N * g.npad * sizeof(T)));
+        HIP_TRY(dGt.alloc(N * sizeof(TA)));
+        HIP_TRY(dGs.alloc(N * sizeof(TA)));
+        HIP_TRY(dGa2.alloc(N * sizeof(TA)));
+        HIP_TRY(dGs2.alloc(N * sizeof(TA)));
+        HIP_TRY(dLam0.alloc((size_t)N * (L + 1) * sizeof(TA)));
+        for (int i = 0; i + 1 < stride; ++i) {
+            seg.emplace_back(new DevBuf());
+            HIP_TRY(seg.back()->alloc(F * sizeof(TA)));
+        }
+        backward_ready = true;
+        return RTE_OK;
     }
-    const int nseg = (n + stride - 1) / stride;
-    for (int j = nseg - 1; j >= 0; --j) {
-        const int jc = j * stride, end = std::min(n, jc + stride);
-        // A W_q for q in [jc, end-2]: stored (stride 1) or recomputed from W_jc
-        auto aw_of = [&](int qq) -> const TA * {
-            if (qq < 0) return nullptr;
-            if (stride == 1 || (qq + 1) % stride == 0) return ckpt[qq]->as<TA>();
-            return seg[qq - jc]->as<TA>();
-        };
-        if (stride > 1 && end - 1 > jc) {
-            if (jc == 0)
-                HIP_TRY(hipMemcpyAsync(dW.p, dS0.p, F * sizeof(T), hipMemcpyDeviceToDevice, st));
-            else
-                HIP_TRY(launch_make_wq<T, TA>(L, g, sc, gan, sv.mus, aw_of(jc - 1), dW.as<T>(), st));
-            for (int qq = jc; qq <= end - 2; ++qq) {
-                sv.mode = 3;
-                sv.aw_store = seg[qq - jc]->as<TA>();
-                HIP_TRY(hipMemcpyAsync(svd, &sv, sizeof sv, hipMemcpyHostToDevice, st));
-                HIP_TRY(launch_prop<T, TA>(L, RTE_PATH_CACHED, 1, pa, svd, 1, st));
-                HIP_TRY(launch_combine<T, TA>(L, ca, svd, 1, st));
+
+    // d(cot . E)/d mu_a, d mu_s for every voxel
+    int backward(const double *cot_host, double *dmu_a, double *dmu_s)
+    {
+        int rc;
+        if ((rc = prepare_backward())) return rc;
+        hipStream_t st = ctx->stream;
+        const rte_grid &grid = m->grid;
+        std::vector<TA> cot(P);
+        for (int i = 0; i < P; ++i) cot[i] = TA(cot_host[i]);
+        HIP_TRY(hipMemcpyAsync(dCot.p, cot.data(), sizeof(TA) * P, hipMemcpyHostToDevice, st));
+        HIP_TRY(launch_fill<T>(dPair.as<T>(), (size_t)N * g.npad, T(0), st));
+        HIP_TRY(launch_fill<TA>(dGt.as<TA>(), N, TA(0), st));
+        HIP_TRY(launch_fill<TA>(dGs.as<TA>(), N, TA(0), st));
+        HIP_TRY(launch_fill<TA>(dGa2.as<TA>(), N, TA(0), st));
+        HIP_TRY(launch_fill<TA>(dGs2.as<TA>(), N, TA(0), st));
+        HIP_TRY(launch_adj_init<T, TA>(L, g, dCot.as<TA>(), sc, dCoefE.as<TA>(), dX.as<T>(), st));
+        // the anisotropy exactly as the forward scatter used it (compute precision)
+        const TA gan = TA(T(m->g));
+        AdjCombArgs<T, TA> aa;
+        std::memset(&aa, 0, sizeof aa);
+        aa.g = g;
+        aa.nchunk = nch;
+        aa.mom = ca.mom;
+        aa.self_scale = ca.self_scale;
+        aa.dV = ca.dV;
+        for (int l = 0; l <= kMaxL; ++l) aa.coefE[l] = ca.coefE[l];
+        for (int a = 0; a < kMaxK; ++a) aa.nsc[a] = ca.nsc[a];
+        aa.X = dX.as<T>();
+        aa.Wq = dWq.as<T>();
+        aa.part = dPart.as<T>();
+        aa.mut = sv.mut;
+        aa.mus = sv.mus;
+        aa.g_aniso = gan;
+        aa.cot = dCot.as<TA>();
+        aa.gt = dGt.as<TA>();
+        aa.gs = dGs.as<TA>();
+        SolveDev<T, TA> *svd = dSv.as<SolveDev<T, TA>>();
+        const int nseg = (n + stride - 1) / stride;
+        for (int j = nseg - 1; j >= 0; --j) {
+            const int jc = j * stride, end = std::min(n, jc + stride);
+            // A W_q for q in [jc, end-2]: stored (stride 1) or recomputed from W_jc
+            auto aw_of = [&](int qq) -> const TA * {
+                if (qq < 0) return nullptr;
+                if (stride == 1 || (qq + 1) % stride == 0) return ckpt[qq]->as<TA>();
+                return seg[qq - jc]->as<TA>();
+            };
+            if (stride > 1 && end - 1 > jc) {
+                if (jc == 0)
+                    HIP_TRY(hipMemcpyAsync(dW.p, dS0.p, F * sizeof(T), hipMemcpyDeviceToDevice, st));
+                else
+                    HIP_TRY(launch_make_wq<T, TA>(L, g, sc, gan, sv.mus, aw_of(jc - 1), dW.as<T>(), st));
+                for (int qq = jc; qq <= end - 2; ++qq) {
+                    SolveDev<T, TA> s2 = sv;
+                    s2.mode = 3;
+                    s2.aw_store = seg[qq - jc]->as<TA>();
+                    HIP_TRY(hipMemcpyAsync(svd, &s2, sizeof s2, hipMemcpyHostToDevice, st));
+                    HIP_TRY(launch_prop<T, TA>(L, RTE_PATH_CACHED, 1, pa, svd, 1, st));
+                    HIP_TRY(launch_combine<T, TA>(L, ca, svd, 1, st));
+                }
+            }
+            for (int qq = end - 1; qq >= jc; --qq) {
+                const TA *awp = aw_of(qq - 1);
+                if (qq == 0)
+                    HIP_TRY(hipMemcpyAsync(dWq.p, dS0.p, F * sizeof(T), hipMemcpyDeviceToDevice, st));
+                else
+                    HIP_TRY(launch_make_wq<T, TA>(L, g, sc, gan, sv.mus, awp, dWq.as<T>(), st));
+                HIP_TRY(launch_adj_prop<T, TA>(L, pa, dX.as<T>(), dWq.as<T>(), dPart.as<T>(), dPair.as<T>(), sv.ktab,
+                                               st));
+                aa.aw_prev = awp;
+                aa.lam0 = (qq == 0) ? dLam0.as<TA>() : nullptr;
+                HIP_TRY(launch_adj_combine<T, TA>(L, aa, st));
             }
         }
-        for (int qq = end - 1; qq >= jc; --qq) {
-            const TA *awp = aw_of(qq - 1);
-            if (qq == 0)
-                HIP_TRY(hipMemcpyAsync(dWq.p, dS0.p, F * sizeof(T), hipMemcpyDeviceToDevice, st));
-            else
-                HIP_TRY(launch_make_wq<T, TA>(L, g, sc, gan, sv.mus, awp, dWq.as<T>(), st));
-            HIP_TRY(launch_adj_prop<T, TA>(L, pa, dX.as<T>(), dWq.as<T>(), dPart.as<T>(), dPair.as<T>(), sv.ktab, st));
-            aa.aw_prev = awp;
-            aa.lam0 = (qq == 0) ? dLam0.as<TA>() : nullptr;
-            HIP_TRY(launch_adj_combine<T, TA>(L, aa, st));
+        HIP_TRY(launch_backproject<T, TA>(g, nch, clen, pa.thr2, dPair.as<T>(), TA(-dV), dGt.as<TA>(), st));
+        HIP_TRY(launch_src_grad<TA>(L, grid.nx, grid.ny, grid.nz, dx, dy, dz, grid.xmin, grid.ymin, m->g, bd,
+                                    dLam0.as<TA>(), dCot.as<TA>(), dGa2.as<TA>(), dGs2.as<TA>(), st));
+        std::vector<TA> gt(N), gs(N), ga2(N), gs2(N);
+        HIP_TRY(hipMemcpyAsync(gt.data(), dGt.p, sizeof(TA) * N, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(gs.data(), dGs.p, sizeof(TA) * N, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(ga2.data(), dGa2.p, sizeof(TA) * N, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(gs2.data(), dGs2.p, sizeof(TA) * N, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        for (int v = 0; v < N; ++v) {
+            if (dmu_a) dmu_a[v] = (double)gt[v] + (double)ga2[v];
+            if (dmu_s) dmu_s[v] = (double)gt[v] + (double)gs[v] + (double)gs2[v];
         }
+        return RTE_OK;
     }
-    HIP_TRY(launch_backproject<T, TA>(g, nch, clen, pa.thr2, dPair.as<T>(), TA(-dV), dGt.as<TA>(), st));
-    HIP_TRY(launch_src_grad<TA>(L, grid.nx, grid.ny, grid.nz, dx, dy, dz, grid.xmin, grid.ymin, m->g, bd,
-                                dLam0.as<TA>(), dCot.as<TA>(), dGa2.as<TA>(), dGs2.as<TA>(), st));
-    HIP_TRY(hipEventRecord(ctx->ev1, st));
-    std::vector<TA> gt(N), gs(N), ga2(N), gs2(N);
-    HIP_TRY(hipMemcpyAsync(gt.data(), dGt.p, sizeof(TA) * N, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(gs.data(), dGs.p, sizeof(TA) * N, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(ga2.data(), dGa2.p, sizeof(TA) * N, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipMemcpyAsync(gs2.data(), dGs2.p, sizeof(TA) * N, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    for (int v = 0; v < N; ++v) {
-        if (dmu_a) dmu_a[v] = (double)gt[v] + (double)ga2[v];
-        if (dmu_s) dmu_s[v] = (double)gt[v] + (double)gs[v] + (double)gs2[v];
-    }
-    if (stats) {
+
+    void fill_stats(rte_stats *stats)
+    {
+        if (!stats) return;
         float ms = 0.f;
-        HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        (void)hipEventRecord(ctx->ev1, ctx->stream);
+        (void)hipEventSynchronize(ctx->ev1);
+        if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) != hipSuccess) ms = 0.f;
         std::memset(stats, 0, sizeof *stats);
         stats->terms = h.n;
         stats->lambda_last = h.lam;
@@ -1259,13 +1282,10 @@ static int vjp_impl(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte
         stats->ballistic_total = h.ball;
         stats->gpu_ms = ms;
     }
-    (void)n;
-    return RTE_OK;
-}
+};
 
-static int vjp_entry(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte_params *params,
-                     const rte_grad_opts *opts, const double *cot, const double *target, double *loss,
-                     double *exitance, double *dmu_a, double *dmu_s, rte_stats *stats)
+static int grad_check(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte_params *params,
+                      const rte_grad_opts *opts, int *stride)
 {
     if (!ctx || !m || !beam || !params) return fail(RTE_E_ARG, "gradient: NULL argument");
     if (m->ctx != ctx) return fail(RTE_E_ARG, "gradient: medium belongs to another context");
@@ -1273,17 +1293,37 @@ static int vjp_entry(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rt
     if ((rc = check_params(params))) return rc;
     if (params->path_mode != RTE_PATH_CACHED)
         return fail(RTE_E_UNSUPP, "gradient: needs the cached path tables (path_mode RTE_PATH_CACHED)");
-    const int stride = opts ? opts->checkpoint_stride : 1;
-    if (stride < 1) return fail(RTE_E_ARG, "checkpoint_stride must be >= 1");
-    if ((rc = set_device(ctx))) return rc;
-    switch (params->precision) {
-    case RTE_PREC_FP64:
-        return vjp_impl<double, double>(ctx, m, beam, params, stride, cot, target, loss, exitance, dmu_a, dmu_s, stats);
-    case RTE_PREC_MIXED:
-        return vjp_impl<float, double>(ctx, m, beam, params, stride, cot, target, loss, exitance, dmu_a, dmu_s, stats);
-    default:
+    if (params->precision != RTE_PREC_FP64 && params->precision != RTE_PREC_MIXED)
         return fail(RTE_E_UNSUPP, "gradient: precision must be RTE_PREC_FP64 or RTE_PREC_MIXED");
+    *stride = opts ? opts->checkpoint_stride : 1;
+    if (*stride < 1) return fail(RTE_E_ARG, "checkpoint_stride must be >= 1");
+    return set_device(ctx);
+}
+
+template <typename T, typename TA>
+static int vjp_run(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte_params *p, int stride,
+                   const double *cot_in, const double *target, double *loss, double *exitance, double *dmu_a,
+                   double *dmu_s, rte_stats *stats)
+{
+    AdjointRun<T, TA> run;
+    int rc;
+    if ((rc = run.forward(ctx, m, beam, p, stride))) return rc;
+    std::vector<double> cot(run.P);
+    double lsum = 0.0;
+    for (int i = 0; i < run.P; ++i) {
+        const double e = (double)run.E[i];
+        if (exitance) exitance[i] = e;
+        if (target) {
+            lsum += (e - target[i]) * (e - target[i]);
+            cot[i] = 2.0 * (e - target[i]);
+        } else {
+            cot[i] = cot_in[i];
+        }
     }
+    if (loss) *loss = lsum;
+    if ((rc = run.backward(cot.data(), dmu_a, dmu_s))) return rc;
+    run.fill_stats(stats);
+    return RTE_OK;
 }
 
 extern "C" int rte_exitance_vjp(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte_params *params,
@@ -1291,7 +1331,13 @@ extern "C" int rte_exitance_vjp(rte_ctx *ctx, rte_medium *m, const rte_beam *bea
                                 double *dmu_s, rte_stats *stats)
 {
     if (!cotangent) return fail(RTE_E_ARG, "rte_exitance_vjp: cotangent is NULL");
-    return vjp_entry(ctx, m, beam, params, opts, cotangent, nullptr, nullptr, exitance, dmu_a, dmu_s, stats);
+    int stride, rc;
+    if ((rc = grad_check(ctx, m, beam, params, opts, &stride))) return rc;
+    if (params->precision == RTE_PREC_FP64)
+        return vjp_run<double, double>(ctx, m, beam, params, stride, cotangent, nullptr, nullptr, exitance, dmu_a,
+                                       dmu_s, stats);
+    return vjp_run<float, double>(ctx, m, beam, params, stride, cotangent, nullptr, nullptr, exitance, dmu_a, dmu_s,
+                                  stats);
 }
 
 extern "C" int rte_gradient(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte_params *params,
@@ -1299,7 +1345,102 @@ extern "C" int rte_gradient(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, c
                             double *dmu_a, double *dmu_s, rte_stats *stats)
 {
     if (!target) return fail(RTE_E_ARG, "rte_gradient: target is NULL");
-    return vjp_entry(ctx, m, beam, params, opts, nullptr, target, loss, exitance, dmu_a, dmu_s, stats);
+    int stride, rc;
+    if ((rc = grad_check(ctx, m, beam, params, opts, &stride))) return rc;
+    if (params->precision == RTE_PREC_FP64)
+        return vjp_run<double, double>(ctx, m, beam, params, stride, nullptr, target, loss, exitance, dmu_a, dmu_s,
+                                       stats);
+    return vjp_run<float, double>(ctx, m, beam, params, stride, nullptr, target, loss, exitance, dmu_a, dmu_s, stats);
+}
+
+static int gram_mfma(rte_ctx *ctx, int P, int n, const float *J, const float *w, float *G);
+
+// ---- Fisher information ---------------------------------------------------
+// J[p][a] = d E_p / d theta_a with theta_a = mu_a (or mu_s) on the voxels of
+// partition a, one backward pass per detector; F = J^T diag(1/E) J on FP32
+// MFMA (detectors with E = 0 skipped, FisherInfo_IO/src/rte_fim.cu:318-324);
+// lower Cholesky factor of F in FP64.
+template <typename T, typename TA>
+static int fisher_run(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte_params *p, int stride,
+                      const int *part, int nparam, int wrt, double *Fout, double *chol, double *Jout, double *exitance,
+                      rte_stats *stats)
+{
+    AdjointRun<T, TA> run;
+    int rc;
+    if ((rc = run.forward(ctx, m, beam, p, stride))) return rc;
+    const int P = run.P, N = run.N;
+    std::vector<double> J((size_t)P * nparam, 0.0), cot(P, 0.0), ga(N), gs(N);
+    for (int pix = 0; pix < P; ++pix) {
+        if (exitance) exitance[pix] = (double)run.E[pix];
+        if (!((double)run.E[pix] != 0.0)) continue;
+        cot.assign(P, 0.0);
+        cot[pix] = 1.0;
+        if ((rc = run.backward(cot.data(), ga.data(), gs.data()))) return rc;
+        const std::vector<double> &gv = wrt == 0 ? ga : gs;
+        for (int v = 0; v < N; ++v)
+            if (part[v] >= 0) J[(size_t)pix * nparam + part[v]] += gv[v];
+    }
+    if (Jout) std::memcpy(Jout, J.data(), sizeof(double) * J.size());
+    // Gram on MFMA (FP32 inputs, FP32 accumulation)
+    std::vector<float> Jf((size_t)P * nparam), wf(P);
+    for (size_t i = 0; i < Jf.size(); ++i) Jf[i] = (float)J[i];
+    for (int pix = 0; pix < P; ++pix) wf[pix] = (double)run.E[pix] != 0.0 ? (float)(1.0 / (double)run.E[pix]) : 0.f;
+    std::vector<float> Ff((size_t)nparam * nparam);
+    if ((rc = gram_mfma(ctx, P, nparam, Jf.data(), wf.data(), Ff.data()))) return rc;
+    // the two triangles come from different MFMA tiles: symmetrise
+    for (int a = 0; a < nparam; ++a)
+        for (int b = 0; b < nparam; ++b)
+            Fout[(size_t)a * nparam + b] = 0.5 * ((double)Ff[(size_t)a * nparam + b] + (double)Ff[(size_t)b * nparam + a]);
+    if (chol) {
+        // FP64 Cholesky F = C C^T (lower); fails if F is not positive definite
+        std::vector<double> C((size_t)nparam * nparam, 0.0);
+        for (int i = 0; i < nparam; ++i)
+            for (int j = 0; j <= i; ++j) {
+                double s = Fout[(size_t)i * nparam + j];
+                for (int k = 0; k < j; ++k) s -= C[(size_t)i * nparam + k] * C[(size_t)j * nparam + k];
+                if (i == j) {
+                    if (!(s > 0.0)) return fail(RTE_E_ARG, "Fisher matrix is not positive definite (pivot %d)", i);
+                    C[(size_t)i * nparam + i] = std::sqrt(s);
+                } else {
+                    C[(size_t)i * nparam + j] = s / C[(size_t)j * nparam + j];
+                }
+            }
+        std::memcpy(chol, C.data(), sizeof(double) * C.size());
+    }
+    run.fill_stats(stats);
+    return RTE_OK;
+}
+
+extern "C" int rte_fisher(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte_params *params,
+                          const rte_grad_opts *opts, const int *partition, int nparam, int wrt, double *F,
+                          double *chol, double *J, double *exitance, rte_stats *stats)
+{
+    if (!partition || !F || nparam < 1) return fail(RTE_E_ARG, "rte_fisher: bad partition/nparam/F");
+    if (wrt != 0 && wrt != 1) return fail(RTE_E_ARG, "rte_fisher: wrt must be 0 (mu_a) or 1 (mu_s)");
+    int stride, rc;
+    if ((rc = grad_check(ctx, m, beam, params, opts, &stride))) return rc;
+    for (int v = 0; v < m->N; ++v)
+        if (partition[v] >= nparam) return fail(RTE_E_ARG, "rte_fisher: partition[%d] = %d >= nparam", v, partition[v]);
+    if (params->precision == RTE_PREC_FP64)
+        return fisher_run<double, double>(ctx, m, beam, params, stride, partition, nparam, wrt, F, chol, J, exitance,
+                                          stats);
+    return fisher_run<float, double>(ctx, m, beam, params, stride, partition, nparam, wrt, F, chol, J, exitance,
+                                     stats);
+}
+
+// Gram G = J^T diag(w) J of a P x n matrix on MFMA (rte_kernels.hip: k_gram)
+static int gram_mfma(rte_ctx *ctx, int P, int n, const float *J, const float *w, float *G)
+{
+    DevBuf dJ, dw, dG;
+    HIP_TRY(dJ.alloc(sizeof(float) * (size_t)P * n));
+    HIP_TRY(dw.alloc(sizeof(float) * P));
+    HIP_TRY(dG.alloc(sizeof(float) * (size_t)n * n));
+    HIP_TRY(hipMemcpyAsync(dJ.p, J, sizeof(float) * (size_t)P * n, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(dw.p, w, sizeof(float) * P, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(launch_gram(P, n, dJ.as<float>(), dw.as<float>(), dG.as<float>(), ctx->stream));
+    HIP_TRY(hipMemcpyAsync(G, dG.p, sizeof(float) * (size_t)n * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return RTE_OK;
 }
 
 // ---------------------------------------------------------------------------

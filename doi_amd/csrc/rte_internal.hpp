@@ -120,6 +120,8 @@ hipError_t launch_to_coef_major(int K, int N, const Scales &sc, const TA *in, do
 hipError_t launch_scatter_op(int L, int N, double g, const double *mus, double *W, hipStream_t st);
 hipError_t launch_exitance_op(int L, int nx, int ny, int nz, const double *coefE, const double *w,
                               double *img, hipStream_t st);
+// G = J^T diag(w) J on MFMA (J: P x n row-major FP32)
+hipError_t launch_gram(int P, int n, const float *J, const float *w, float *G, hipStream_t st);
 hipError_t launch_tau_list(const GridDev<float> &g, const float *mut, int np, const int *dst,
                            const int *src, double *tau, hipStream_t st);
 

@@ -438,6 +438,44 @@ __global__ void k_tau_list(GridDev<float> g, const float *__restrict__ mut, int 
 }
 
 // ---------------------------------------------------------------------------
+// Fisher Gram G = J^T diag(w) J (J: P x n, row-major) on MFMA:
+// v_mfma_f32_16x16x4f32, one wave per 16x16 tile of G, four detectors per
+// step.  Operand layout of the instruction (wave64): lane l holds
+// A[l % 16][l / 16] and B[l / 16][l % 16]; the 4 results of lane l are
+// D[4 (l / 16) + r][l % 16].  grid = (ceil(n/16), ceil(n/16)), block = 64.
+// ---------------------------------------------------------------------------
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(kWave) void k_gram(int P, int n, const float *__restrict__ J,
+                                               const float *__restrict__ w, float *__restrict__ G)
+{
+    const int lane = threadIdx.x;
+    const int r0 = blockIdx.x * 16, c0 = blockIdx.y * 16;
+    const int ia = r0 + (lane & 15), kk = lane >> 4;
+    const int jb = c0 + (lane & 15);
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int p0 = 0; p0 < P; p0 += 4) {
+        const int pp = p0 + kk;
+        const bool okp = pp < P;
+        const float a = (okp && ia < n) ? J[(size_t)pp * n + ia] : 0.f;
+        const float b = (okp && jb < n) ? w[pp] * J[(size_t)pp * n + jb] : 0.f;
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int row = r0 + 4 * (lane >> 4) + r, col = c0 + (lane & 15);
+        if (row < n && col < n) G[(size_t)row * n + col] = acc[r];
+    }
+}
+
+hipError_t launch_gram(int P, int n, const float *J, const float *w, float *G, hipStream_t st)
+{
+    dim3 grid((n + 15) / 16, (n + 15) / 16), block(kWave);
+    hipLaunchKernelGGL(k_gram, grid, block, 0, st, P, n, J, w, G);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // host-side launchers (dispatch over L)
 // ---------------------------------------------------------------------------
 #define RTE_DISPATCH_L(L, ...)                          \

@@ -92,6 +92,8 @@ SIGNATURES = {
                                    _dp, _dp, C.POINTER(Stats)]),
     "rte_gradient": (C.c_int, [_vp, _vp, C.POINTER(Beam), C.POINTER(Params), C.POINTER(GradOpts), _dp, _dp, _dp,
                                _dp, _dp, C.POINTER(Stats)]),
+    "rte_fisher": (C.c_int, [_vp, _vp, C.POINTER(Beam), C.POINTER(Params), C.POINTER(GradOpts), _ip, C.c_int,
+                             C.c_int, _dp, _dp, _dp, _dp, C.POINTER(Stats)]),
     "rte_propagate": (C.c_int, [_vp, C.POINTER(Grid), C.POINTER(Optics), C.c_int, C.c_double,
                                 C.c_int, C.c_int, _dp, _dp]),
     "rte_scatter": (C.c_int, [_vp, C.POINTER(Grid), C.POINTER(Optics), C.c_int, _dp]),
@@ -249,6 +251,19 @@ class Context:
                                   C.byref(GradOpts(checkpoint_stride)), _p(tgt), C.byref(loss), _p(img), _p(ga),
                                   _p(gs), C.byref(st)))
         return loss.value, img.reshape(ny, nx), ga, gs, dict(terms=st.terms, gpu_ms=st.gpu_ms)
+
+    def fisher(self, medium, beam, params, partition, nparam, wrt=0, checkpoint_stride=1):
+        nx, ny, _ = medium.shape
+        part = np.ascontiguousarray(np.ravel(partition), dtype=np.int32)
+        F = np.zeros((nparam, nparam))
+        Cf = np.zeros((nparam, nparam))
+        J = np.zeros((nx * ny, nparam))
+        img = np.zeros(nx * ny)
+        st = Stats()
+        _check(lib().rte_fisher(self.h, medium.h, C.byref(Beam(*beam)), C.byref(params),
+                                C.byref(GradOpts(checkpoint_stride)), part.ctypes.data_as(_ip), nparam, wrt,
+                                _p(F), _p(Cf), _p(J), _p(img), C.byref(st)))
+        return F, Cf, J, img.reshape(ny, nx), dict(terms=st.terms, gpu_ms=st.gpu_ms)
 
     # ---- operators -------------------------------------------------------
     def propagate(self, shape, bounds, mu_a, mu_s, g, W, L, prop_thresh=0.0, precision=PREC_FP64,

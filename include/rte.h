@@ -187,6 +187,17 @@ int rte_gradient(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte_pa
                  const rte_grad_opts *opts, const double *target, double *loss, double *exitance,
                  double *dmu_a, double *dmu_s, rte_stats *stats);
 
+/* Fisher information of Poisson detectors for block parameters
+ * (FisherInfo_IO/src/rte_fim.cu:316-331, FisherInfo_IO/src/inverse_rte.cu:301-316):
+ * theta_a = mu_a (wrt = 0) or mu_s (wrt = 1) on the voxels with partition[v] == a
+ * (-1: not a parameter); J[p][a] = dE_p/dtheta_a (nx*ny x nparam, may be NULL),
+ * F = J^T diag(1/E) J (nparam x nparam, detectors with E = 0 skipped) formed on FP32
+ * MFMA, chol = its lower Cholesky factor in FP64 (may be NULL; RTE_E_ARG if F is not
+ * positive definite). */
+int rte_fisher(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte_params *params,
+               const rte_grad_opts *opts, const int *partition, int nparam, int wrt, double *F,
+               double *chol, double *J, double *exitance, rte_stats *stats);
+
 /* ---- operators (for operator-level parity) ------------------------------ */
 /* A: propagation incl. self-voxel block and dV (CPU package/src/Kernels_selfsub.c:450-538,
  * GPU package/src/kernels.cu:148-175,286-381).  W, out: K*N doubles. */

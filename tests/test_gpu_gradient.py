@@ -101,10 +101,47 @@ def test_loss_gradient_and_mixed_precision(ctx):
     loss, img, ga, gs, st = ctx.gradient(med, BEAM, p64, target)
     assert abs(loss - np.sum((img - target) ** 2)) < 1e-12 * max(loss, 1e-30)
     _, ga2, gs2, _ = ctx.exitance_vjp(med, BEAM, p64, 2 * (img - target))
-    assert np.allclose(ga, ga2, rtol=1e-12, atol=0) and np.allclose(gs, gs2, rtol=1e-12, atol=0)
+    sc0 = max(np.max(np.abs(ga)), np.max(np.abs(gs)))
+    assert np.max(np.abs(ga - ga2)) < 1e-12 * sc0 and np.max(np.abs(gs - gs2)) < 1e-12 * sc0
     pm = rte.make_params(L, precision=rte.PREC_MIXED, stop_rule=rte.STOP_DELTA, tol=1e-7, max_terms=30)
     lm, _, gam, gsm, stm = ctx.gradient(med, BEAM, pm, target)
     assert stm["terms"] == st["terms"]
     sc = max(np.max(np.abs(ga)), np.max(np.abs(gs)))
     assert np.max(np.abs(gam - ga)) < 1e-4 * sc and np.max(np.abs(gsm - gs)) < 1e-4 * sc
     med.close()
+
+
+def test_fisher_information(ctx):
+    """J against central differences of block parameters, F = J^T diag(1/E) J
+    (FP32 MFMA) against an FP64 host Gram of the returned J, Cholesky factor."""
+    shape = (6, 6, 5)
+    bounds = bounds_for(shape)
+    ma, ms = hetero_maps(shape, seed=17, sigma=0.3)
+    # 3 x 3 x 2 blocks of 2 x 2 x (3|2) voxels -> 18 parameters (not a multiple of 16)
+    i, j, k = np.meshgrid(np.arange(6), np.arange(6), np.arange(5), indexing="ij")
+    part = ((i // 2) + 3 * (j // 2) + 9 * (k // 3)).transpose(2, 1, 0).ravel()
+    nparam = 18
+    med = ctx.medium(shape, bounds, ma, ms, G)
+    p = rte.make_params(L, precision=rte.PREC_FP64, stop_rule=rte.STOP_DELTA, tol=1e-8, max_terms=40)
+    F, Cf, J, img, st = ctx.fisher(med, BEAM, p, part, nparam, wrt=0)
+    n = st["terms"]
+    med.close()
+    E = img.ravel()
+    assert np.all(E > 0)
+    # J columns by central differences (n fixed)
+    pf = rte.make_params(L, precision=rte.PREC_FP64, stop_rule=rte.STOP_DELTA, tol=0.0, max_terms=n)
+    for a in (0, 4, 13, 17):
+        h = 1e-4 * np.mean(ma)
+        up, dn = ma.copy(), ma.copy()
+        up[part == a] += h
+        dn[part == a] -= h
+        ep, _ = ctx.solve(shape, bounds, up, ms, G, BEAM, pf)
+        em, _ = ctx.solve(shape, bounds, dn, ms, G, BEAM, pf)
+        fd = (ep - em).ravel() / (2 * h)
+        assert np.max(np.abs(J[:, a] - fd)) < 2e-6 * np.max(np.abs(J)) + 1e-5 * np.max(np.abs(fd))
+    # Gram on MFMA against FP64 on the host
+    F64 = J.T @ (J / E[:, None])
+    assert np.max(np.abs(F - F64)) < 2e-5 * np.max(np.abs(F64))
+    assert np.array_equal(F, F.T)
+    assert np.allclose(np.tril(Cf), Cf)
+    assert np.max(np.abs(Cf @ Cf.T - F)) < 1e-12 * np.max(np.abs(F))
