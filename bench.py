@@ -8,6 +8,7 @@ mu_a inclusions, 5 pencil beams per medium -> 160 solves, Neumann series to a
 in HBM before the timed region; every step rebuilds their path tables.
 
     python bench.py [--gpus 1] [--steps 2] [--warmup 1]
+    python bench.py --serve [--requests 2000] [--rate 200]   # serving stream (config 4)
 
 Prints one JSON line.  One GPU only: --gpus N > 1 exits with a message.
 """
@@ -56,6 +57,101 @@ def cpu_baseline(shape, bounds, mu_a, mu_s, g, L, terms, budget_s=12.0):
                        % (done, N, el, per_prop, terms))
 
 
+def serve(args):
+    """Serving stream: seeded Poisson arrivals of mixed-shape solves submitted
+    in real time to the continuous-batching server; throughput, p50/p99
+    latency, and the plain one-at-a-time FP32 path timed on a stratified
+    sample of the same stream."""
+    import threading
+    from doi_amd import media, rte
+
+    reqs = media.serving_requests(args.requests, seed=4, rate=args.rate)
+    classes = sorted(set((r["shape"][0], r["L"]) for r in reqs))
+    ctx = rte.Context(0)
+    srv = rte.Server(ctx)
+    # warm-up: one solve of every shape class (per-shape tables, buckets)
+    warm = {}
+    for r in reqs:
+        warm.setdefault((r["shape"][0], r["L"]), r)
+    for r in warm.values():
+        srv.wait(srv.submit(r["shape"], r["bounds"], r["mu_a"], r["mu_s"], r["g"], r["beam"], r["params"]))
+    log("serving %d requests at %.0f/s, classes %s" % (len(reqs), args.rate, classes))
+    ids = [None] * len(reqs)
+    results = [None] * len(reqs)
+    ready = threading.Semaphore(0)
+
+    def collect():
+        for k in range(len(reqs)):
+            ready.acquire()
+            results[k] = srv.wait(ids[k])
+            if (k + 1) % 250 == 0:
+                log("%d done" % (k + 1))
+
+    th = threading.Thread(target=collect)
+    th.start()
+    t0 = time.perf_counter()
+    for k, r in enumerate(reqs):
+        dt = t0 + r["t"] - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        while True:
+            try:
+                ids[k] = srv.submit(r["shape"], r["bounds"], r["mu_a"], r["mu_s"], r["g"], r["beam"], r["params"])
+                break
+            except rte.RteError as e:
+                if e.code != rte.E_BUSY:
+                    raise
+                time.sleep(0.001)
+        ready.release()
+    th.join()
+    wall = time.perf_counter() - t0
+    info = srv.info()
+    srv.close()
+    lat = np.array([res[2]["latency_ms"] for res in results])
+    served = len(reqs) / wall
+    # plain path on a sample: 2 solves per class, FP32, marched path integrals, one at a time
+    per_class = {}
+    for c in classes:
+        sample = [r for r in reqs if (r["shape"][0], r["L"]) == c][:args.plain_sample]
+        ts = []
+        for r in sample:
+            p = rte.make_params(r["L"], precision=rte.PREC_FP32, stop_rule=rte.STOP_DELTA, tol=1e-6, max_terms=30,
+                                path_mode=rte.PATH_MARCH)
+            t1 = time.perf_counter()
+            ctx.solve(r["shape"], r["bounds"], r["mu_a"], r["mu_s"], r["g"], r["beam"], p)
+            ts.append(time.perf_counter() - t1)
+        per_class["%d^3 L%d" % c] = float(np.mean(ts))
+        log("plain %d^3 L%d: %.4f s" % (c[0], c[1], per_class["%d^3 L%d" % c]))
+    t_plain = sum(per_class["%d^3 L%d" % (r["shape"][0], r["L"])] for r in reqs)
+    plain = len(reqs) / t_plain
+    res = {
+        "metric": "RTE serving stream: solves/s + p50/p99 latency (mixed grids 8-32^3, L 3/5)",
+        "value": served,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": len(warm),
+        "ms_per_step": 1000.0 * wall,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (fp64 running sums)",
+        "data": "synthetic (seeded serving stream, config 4)",
+        "config": {"workload": "config4: %d requests, Poisson %.0f/s, edges {8,12,16,24,32}, L {3,5}, "
+                               "g U[0.7,0.9], mu_s U[0.5,1.5], mu_a U[0.005,0.05], sigma 0.2, tol 1e-6 cap 30"
+                               % (len(reqs), args.rate)},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "max": float(lat.max())},
+        "arrival_span_s": reqs[-1]["t"],
+        "server": {"steps": info["steps"], "buckets": info["buckets"], "failed": info["failed"]},
+        "plain_fp32_solves_per_s": plain,
+        "served_over_plain": served / plain,
+        "plain_s_per_solve_by_class": per_class,
+    }
+    print(json.dumps(res), flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +164,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--group", type=int, default=0, help="max beams per propagation group (0 = library default)")
+    ap.add_argument("--serve", action="store_true", help="run the serving stream instead of the batch workload")
+    ap.add_argument("--requests", type=int, default=2000)
+    ap.add_argument("--rate", type=float, default=200.0)
+    ap.add_argument("--plain-sample", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (independent solves "
@@ -75,6 +175,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.serve:
+        return serve(args)
     from doi_amd import media, rte
 
     n, L = args.n, args.L

@@ -271,7 +271,7 @@ struct rte_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool prof = false;
-    int max_rhs = kMaxNR;
+    int max_rhs = 2;  // beams per propagation group (1..kMaxNR); 2 measured fastest at 32^3 L=5
     rte_prof pc{};
     std::vector<hipEvent_t> ev_pool;
     std::vector<PendingTiming> pending;
@@ -608,8 +608,9 @@ static int pick_chunks(int nblk, int nsolve, int N)
 }
 
 template <typename T>
-static int ensure_ktab(rte_medium *m, double thr)
+static int ensure_ktab(rte_medium *m, double thr, hipStream_t on = nullptr)
 {
+    hipStream_t st = on ? on : m->ctx->stream;
     DevBuf &tab = sizeof(T) == 4 ? m->ktab_f : m->ktab_d;
     double &tthr = sizeof(T) == 4 ? m->ktab_f_thr : m->ktab_d_thr;
     if (tab.p && tthr == thr) return RTE_OK;
@@ -619,9 +620,13 @@ static int ensure_ktab(rte_medium *m, double thr)
     const int clen = (g.N + nchunk - 1) / nchunk;
     const T *mut = sizeof(T) == 4 ? (const T *)m->mut_f.p : (const T *)m->mut_d.p;
     const T thr2 = thr > 0 ? T(thr * thr) : T(sizeof(T) == 4 ? 1e30 : 1e300);
-    LaunchTimer tm(m->ctx, 1);
-    HIP_TRY(launch_ktab<T>(g, (g.N + clen - 1) / clen, clen, thr2, mut, tab.as<T>(), m->ctx->stream));
-    tm.done((double)g.N * g.N);
+    if (st == m->ctx->stream) {
+        LaunchTimer tm(m->ctx, 1);
+        HIP_TRY(launch_ktab<T>(g, (g.N + clen - 1) / clen, clen, thr2, mut, tab.as<T>(), st));
+        tm.done((double)g.N * g.N);
+    } else {
+        HIP_TRY(launch_ktab<T>(g, (g.N + clen - 1) / clen, clen, thr2, mut, tab.as<T>(), st));
+    }
     tthr = thr;
     return RTE_OK;
 }
@@ -1676,4 +1681,785 @@ extern "C" int rte_path_integrals(rte_ctx *ctx, const rte_grid *grid, const rte_
     rte_medium_destroy(m);
     g_err = keep;
     return rc;
+}
+
+// ===========================================================================
+// Serving: continuous batching.
+//
+// A worker thread owns the device while the server exists.  Requests wait in
+// a FIFO and are granted a solve slot of the bucket of their shape (grid, L,
+// precision, path, stop rule) while the device memory of all slots fits the
+// HBM budget, first come first served.  A slot holds one solve's whole state
+// (optical maps, path table, Neumann field, partial sums, images) and is
+// pooled: when its solve leaves, the next request of that shape reuses it, and
+// idle slots of other shapes are freed only when memory is needed.
+//
+// Each bucket has its own stream and at most one step in flight: joining
+// solves get their maps, path table, source and initial sums, then every
+// resident solve advances one Neumann term in a HIP graph (propagation,
+// self block/scatter/exitance, bottom-layer sums) that is captured once per
+// resident count and replayed; the per-solve descriptors live in a device
+// array at a fixed address that is refreshed before every replay.  The worker
+// polls the buckets' events, so a bucket of small grids runs its terms while a
+// bucket of large grids is still in its own, and the work of all buckets
+// overlaps on the device.  A solve's arithmetic does not depend on the other
+// solves of its step, so its result is the same alone and in any mix.
+// ===========================================================================
+#include <atomic>
+#include <condition_variable>
+#include <deque>
+#include <list>
+#include <mutex>
+#include <thread>
+
+namespace {
+
+enum { REQ_QUEUED = 0, REQ_RUNNING = 1, REQ_DONE = 2, REQ_FAILED = 3, REQ_CANCELLED = 4 };
+
+struct ServReq {
+    long long id = 0;
+    rte_grid grid{};
+    std::vector<double> mua, mus, mutd;  // FP64 maps, mu_t = mu_a + mu_s
+    std::vector<float> mutf, musf;       // FP32 mu_t, mu_s
+    double g = 0;
+    rte_beam beam{};
+    int beam_cols = 0;
+    rte_params params{};
+    std::string key;
+    double t_submit = 0, t_admit = 0;
+    int state = REQ_QUEUED;  // read and written under the server lock
+    int outcome = REQ_DONE;  // set by the worker, published as state under the lock
+    std::atomic<bool> cancel{false};
+    int rc = 0;
+    std::string err;
+    long long footprint = 0;
+    std::vector<double> img;
+    rte_stats st{};
+};
+
+struct SlotBase {
+    ServReq *r = nullptr;  // null: idle
+    int idx = 0;           // row of the bucket's reduction and image arrays
+    long long bytes = 0;
+    DevBuf mua, mus, mutd, mutf, musf, ktab, W, part, pix;
+    HostSolve h{};
+    bool fresh = false, ran = false;
+    void free_memory()
+    {
+        for (DevBuf *b : {&mua, &mus, &mutd, &mutf, &musf, &ktab, &W, &part, &pix}) b->release();
+        bytes = 0;
+    }
+};
+
+struct BucketBase {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_step = nullptr, ev_drain = nullptr;
+    bool step_inflight = false, drain_inflight = false;
+    int init_err = RTE_OK;
+    int cap = 0;
+    long long slot_bytes = 0;
+    std::vector<std::unique_ptr<SlotBase>> pool;  // allocated slots, index = idx
+    std::vector<SlotBase *> joining, live, draining;
+    virtual ~BucketBase() {}
+    virtual int launch(std::vector<ServReq *> &finished) = 0;   // enqueue one step (stream idle)
+    virtual void complete(std::vector<ServReq *> &finished) = 0; // after ev_step
+    virtual void drained(std::vector<ServReq *> &finished) = 0;  // after ev_drain
+    virtual int alloc_slot(SlotBase &s) = 0;
+    int n_busy() const { return (int)(joining.size() + live.size() + draining.size()); }
+    SlotBase *idle_slot()
+    {
+        for (auto &s : pool)
+            if (!s->r && s->bytes) return s.get();
+        return nullptr;
+    }
+    // frees one idle slot's memory; returns the bytes released
+    long long evict_one()
+    {
+        for (auto &s : pool)
+            if (!s->r && s->bytes) {
+                const long long b = s->bytes;
+                s->free_memory();
+                return b;
+            }
+        return 0;
+    }
+    int n_allocated() const
+    {
+        int n = 0;
+        for (auto &s : pool) n += s->bytes ? 1 : 0;
+        return n;
+    }
+};
+
+template <typename T, typename TA>
+struct Bucket : BucketBase {
+    rte_ctx *ctx = nullptr;
+    rte_grid grid{};
+    rte_params p{};
+    int L = 0, K = 0, N = 0, P = 0, nch = 0, clen = 0, max_terms = 0;
+    double dx = 0, dy = 0, dz = 0, thr = -1;
+    GridDev<T> g{};
+    Scales sc{};
+    PropArgs<T> pa{};
+    CombArgs<T, TA> ca{};
+    std::vector<SolveDev<T, TA>> sv;  // per slot index
+    DevBuf d_init, d_beam, d_all, d_red;
+    // pinned host staging: [init sv | all sv | beams], the sums, the images
+    char *h_stage = nullptr;
+    TA *h_red = nullptr;  // [cap][2] term sums, then [cap][2] initial sums
+    TA *h_img = nullptr;  // [cap][P]
+    std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs;  // by resident count
+
+    Bucket(rte_ctx *c, const rte_grid &gr, const rte_params &pr, int capacity, MomentSet *mom)
+        : ctx(c), grid(gr), p(pr)
+    {
+        cap = capacity;
+        L = p.L;
+        K = (L + 1) * (L + 1);
+        g = make_grid<T>(grid);
+        N = g.N;
+        P = grid.nx * grid.ny;
+        dx = (grid.xmax - grid.xmin) / grid.nx;
+        dy = (grid.ymax - grid.ymin) / grid.ny;
+        dz = (grid.zmax - grid.zmin) / grid.nz;
+        thr = p.prop_thresh > 0 ? p.prop_thresh : -1.0;
+        max_terms = p.max_terms > 0 ? p.max_terms : 1000;
+        slot_bytes = rte_solve_footprint(&grid, &p, p.path_mode);
+        // the same source chunking as rte_solve_media: it depends on the grid only
+        const int nch0 = pick_chunks(g.nblk, 1, N);
+        clen = (N + nch0 - 1) / nch0;
+        nch = (N + clen - 1) / clen;
+        sc = make_scales(L);
+        pa.g = g;
+        pa.sh = make_sh_const<T>();
+        pa.nchunk = nch;
+        pa.chunk_len = clen;
+        pa.thr2 = thr > 0 ? T(thr * thr) : T(sizeof(T) == 4 ? 1e30 : 1e300);
+        std::memset(&ca, 0, sizeof ca);
+        ca.g = g;
+        ca.nchunk = nch;
+        ca.op = 0;
+        ca.mom = sizeof(TA) == 4 ? (const TA *)mom->mf.p : (const TA *)mom->md.p;
+        ca.self_scale = TA((M_PI / kAngRes) * (2.0 * M_PI / kAngRes) / (dx * dy * dz));
+        ca.dV = TA(dx * dy * dz);
+        double Q[kMaxL + 1];
+        quad_weights(L, p.exit_quad, Q);
+        for (int l = 0; l <= L; ++l)
+            ca.coefE[l] = TA(2.0 * M_PI * dx * dy / std::sqrt(4.0 * M_PI) * std::sqrt(2.0 * l + 1.0) * Q[l]);
+        for (int a = 0; a < K; ++a) ca.wsum[a] = TA(stop_weight(a));
+        for (int a = 0; a < kMaxK; ++a) ca.nsc[a] = TA(sc.n[a]);
+        sv.resize(cap);
+        for (int i = 0; i < cap; ++i) {
+            pool.emplace_back(new SlotBase);
+            pool.back()->idx = i;
+        }
+        if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) {
+            stream = nullptr;
+            init_err = RTE_E_DEVICE;
+            return;
+        }
+        const size_t svb = sizeof(SolveDev<T, TA>) * cap;
+        if (hipEventCreateWithFlags(&ev_step, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&ev_drain, hipEventDisableTiming) != hipSuccess) {
+            init_err = RTE_E_DEVICE;
+            return;
+        }
+        if (d_init.alloc(svb) != hipSuccess || d_all.alloc(svb) != hipSuccess ||
+            d_beam.alloc(sizeof(BeamDev) * cap) != hipSuccess || d_red.alloc(sizeof(TA) * 2 * cap) != hipSuccess ||
+            hipHostMalloc((void **)&h_stage, 2 * svb + sizeof(BeamDev) * cap, hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void **)&h_red, sizeof(TA) * 4 * cap, hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void **)&h_img, sizeof(TA) * (size_t)P * cap, hipHostMallocDefault) != hipSuccess)
+            init_err = RTE_E_NOMEM;
+    }
+
+    ~Bucket() override
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto &kv : graphs) {
+            (void)hipGraphExecDestroy(kv.second.second);
+            (void)hipGraphDestroy(kv.second.first);
+        }
+        for (auto &s : pool) s->free_memory();
+        if (h_img) (void)hipHostFree(h_img);
+        if (h_red) (void)hipHostFree(h_red);
+        if (h_stage) (void)hipHostFree(h_stage);
+        if (ev_step) (void)hipEventDestroy(ev_step);
+        if (ev_drain) (void)hipEventDestroy(ev_drain);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    SolveDev<T, TA> *h_init() { return reinterpret_cast<SolveDev<T, TA> *>(h_stage); }
+    SolveDev<T, TA> *h_all() { return h_init() + cap; }
+    BeamDev *h_beam() { return reinterpret_cast<BeamDev *>(h_init() + 2 * cap); }
+
+    int alloc_slot(SlotBase &s) override
+    {
+        const size_t n = (size_t)N, F = (size_t)N * K;
+        const bool cached = p.path_mode == RTE_PATH_CACHED;
+        if (s.mua.alloc(n * 8) || s.mus.alloc(n * 8) || s.mutd.alloc(n * 8) || s.mutf.alloc(n * 4) ||
+            s.musf.alloc(n * 4) || (cached && s.ktab.alloc((size_t)N * g.npad * sizeof(T))) ||
+            s.W.alloc(F * sizeof(T)) || s.part.alloc((size_t)nch * F * sizeof(T)) ||
+            s.pix.alloc((size_t)3 * P * sizeof(TA))) {
+            s.free_memory();
+            return fail(RTE_E_NOMEM, "server: device memory for a %dx%dx%d L=%d solve", grid.nx, grid.ny, grid.nz, L);
+        }
+        s.bytes = slot_bytes;
+        return RTE_OK;
+    }
+
+    // maps, path table and descriptors of a joining solve (enqueued on the stream)
+    int setup(SlotBase &s, int k)
+    {
+        ServReq *r = s.r;
+        const size_t n = (size_t)N;
+        HIP_TRY(hipMemcpyAsync(s.mua.p, r->mua.data(), n * 8, hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipMemcpyAsync(s.mus.p, r->mus.data(), n * 8, hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipMemcpyAsync(s.mutd.p, r->mutd.data(), n * 8, hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipMemcpyAsync(s.mutf.p, r->mutf.data(), n * 4, hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipMemcpyAsync(s.musf.p, r->musf.data(), n * 4, hipMemcpyHostToDevice, stream));
+        const T *mut = sizeof(T) == 4 ? (const T *)s.mutf.p : (const T *)s.mutd.p;
+        if (p.path_mode == RTE_PATH_CACHED)
+            HIP_TRY(launch_ktab<T>(g, nch, clen, pa.thr2, mut, s.ktab.template as<T>(), stream));
+        const size_t F = (size_t)N * K;
+        HIP_TRY(launch_fill<T>(s.W.template as<T>(), F, T(0), stream));
+        SolveDev<T, TA> &v = sv[s.idx];
+        std::memset(&v, 0, sizeof v);
+        v.W = s.W.template as<T>();
+        v.part = s.part.template as<T>();
+        v.ktab = p.path_mode == RTE_PATH_CACHED ? s.ktab.template as<T>() : nullptr;
+        v.mut = mut;
+        v.mus = sizeof(T) == 4 ? (const T *)s.musf.p : (const T *)s.mus.p;
+        v.g = T(r->g);
+        v.dE = s.pix.template as<TA>();
+        v.bsum = v.dE + P;
+        v.E = v.dE + 2 * P;
+        v.red = d_red.template as<TA>() + 2 * s.idx;
+        v.mode = 1;
+        h_init()[k] = v;
+        // beam columns and power per column (CPU package/src/Dist.c:1090-1114, Inits.c:56)
+        h_beam()[k] = BeamDev{r->beam.pos_x, r->beam.pos_y, (r->beam.diam + 1.0e-16) / 2.0,
+                              r->beam.mag / (double)r->beam_cols, s.mua.template as<double>(),
+                              s.mus.template as<double>()};
+        s.h = HostSolve{nullptr, 0, 0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0};
+        if (p.stop_rule == RTE_STOP_LAMBDA) {
+            s.h.n = 1;
+            s.h.state = (max_terms <= 1) ? 2 : 0;
+        }
+        s.fresh = true;
+        return RTE_OK;
+    }
+
+    int launch(std::vector<ServReq *> &finished) override
+    {
+        int nj = 0;
+        for (SlotBase *s : joining) {
+            if (s->r->cancel.load()) {
+                s->r->outcome = REQ_CANCELLED;
+                finished.push_back(s->r);
+                s->r = nullptr;
+                continue;
+            }
+            if (const int rc = setup(*s, nj)) return rc;
+            ++nj;
+            live.push_back(s);
+        }
+        joining.clear();
+        if (nj) {
+            HIP_TRY(hipMemcpyAsync(d_init.p, h_init(), sizeof(SolveDev<T, TA>) * nj, hipMemcpyHostToDevice, stream));
+            HIP_TRY(hipMemcpyAsync(d_beam.p, h_beam(), sizeof(BeamDev) * nj, hipMemcpyHostToDevice, stream));
+            HIP_TRY(launch_source<T, TA>(L, g, dx, dy, dz, grid.xmin, grid.ymin, sc, d_beam.as<BeamDev>(),
+                                         d_init.as<SolveDev<T, TA>>(), nj, stream));
+            HIP_TRY(launch_bottom<T, TA>(P, d_init.as<SolveDev<T, TA>>(), nj, stream));
+            HIP_TRY(hipMemcpyAsync(h_red + 2 * cap, d_red.p, sizeof(TA) * 2 * cap, hipMemcpyDeviceToHost, stream));
+        }
+        if (!live.empty()) {
+            // one term of every resident solve
+            const int na = (int)live.size();
+            SolveDev<T, TA> *all = h_all();
+            for (int j = 0; j < na; ++j) {
+                all[j] = sv[live[j]->idx];
+                all[j].mode = live[j]->h.state;
+                live[j]->ran = true;
+            }
+            HIP_TRY(hipMemcpyAsync(d_all.p, all, sizeof(SolveDev<T, TA>) * na, hipMemcpyHostToDevice, stream));
+            auto it = graphs.find(na);
+            if (it == graphs.end()) {
+                HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+                const SolveDev<T, TA> *d = d_all.as<SolveDev<T, TA>>();
+                hipError_t e = launch_prop<T, TA>(L, p.path_mode, 1, pa, d, na, stream);
+                if (e == hipSuccess) e = launch_combine<T, TA>(L, ca, d, na, stream);
+                if (e == hipSuccess) e = launch_bottom<T, TA>(P, d, na, stream);
+                hipGraph_t gr = nullptr;
+                const hipError_t e2 = hipStreamEndCapture(stream, &gr);
+                if (e != hipSuccess || e2 != hipSuccess) {
+                    if (gr) (void)hipGraphDestroy(gr);
+                    return fail(RTE_E_DEVICE, "server: graph capture failed (%s)",
+                                hipGetErrorString(e != hipSuccess ? e : e2));
+                }
+                hipGraphExec_t ex = nullptr;
+                const hipError_t e3 = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+                if (e3 != hipSuccess) {
+                    (void)hipGraphDestroy(gr);
+                    return fail(RTE_E_DEVICE, "server: graph instantiation failed (%s)", hipGetErrorString(e3));
+                }
+                it = graphs.emplace(na, std::make_pair(gr, ex)).first;
+            }
+            HIP_TRY(hipGraphLaunch(it->second.second, stream));
+            HIP_TRY(hipMemcpyAsync(h_red, d_red.p, sizeof(TA) * 2 * cap, hipMemcpyDeviceToHost, stream));
+        }
+        HIP_TRY(hipEventRecord(ev_step, stream));
+        step_inflight = true;
+        return RTE_OK;
+    }
+
+    void complete(std::vector<ServReq *> &finished) override
+    {
+        std::vector<SlotBase *> keep;
+        bool copies = false;
+        for (SlotBase *s : live) {
+            HostSolve &h = s->h;
+            if (s->fresh) {
+                h.ball = (double)h_red[2 * cap + 2 * s->idx];
+                h.T = h.ball;
+                if (p.stop_rule == RTE_STOP_LAMBDA) h.sumOut = (double)h_red[2 * cap + 2 * s->idx + 1];
+                s->fresh = false;
+            }
+            if (s->ran) {
+                advance_series(h, (double)h_red[2 * s->idx], (double)h_red[2 * s->idx + 1], p.stop_rule, p.tol,
+                               max_terms);
+                s->ran = false;
+            }
+            if (s->r->cancel.load()) {
+                s->r->outcome = REQ_CANCELLED;
+                finished.push_back(s->r);
+                s->r = nullptr;
+            } else if (h.state == 3) {
+                // the image is read back asynchronously; the slot stays busy until it lands
+                (void)hipMemcpyAsync(h_img + (size_t)s->idx * P, sv[s->idx].E, sizeof(TA) * P, hipMemcpyDeviceToHost,
+                                     stream);
+                draining.push_back(s);
+                copies = true;
+            } else {
+                keep.push_back(s);
+            }
+        }
+        live.swap(keep);
+        step_inflight = false;
+        if (copies) {
+            (void)hipEventRecord(ev_drain, stream);
+            drain_inflight = true;
+        }
+    }
+
+    void drained(std::vector<ServReq *> &finished) override
+    {
+        for (SlotBase *s : draining) {
+            ServReq *r = s->r;
+            const TA *im = h_img + (size_t)s->idx * P;
+            r->img.assign(im, im + P);
+            std::memset(&r->st, 0, sizeof r->st);
+            r->st.terms = s->h.n;
+            r->st.lambda_last = s->h.lam;
+            r->st.rel_change_last = s->h.rel;
+            r->st.total_exitance = s->h.T;
+            r->st.ballistic_total = s->h.ball;
+            r->outcome = REQ_DONE;
+            finished.push_back(r);
+            s->r = nullptr;
+        }
+        draining.clear();
+        drain_inflight = false;
+    }
+};
+
+}  // namespace
+
+struct rte_server {
+    rte_ctx *ctx = nullptr;
+    rte_server_opts opts{};
+    std::mutex mu;
+    std::condition_variable cv, cv_done;
+    std::deque<ServReq *> queue;                 // submitted, not yet seen by the worker
+    std::map<long long, std::unique_ptr<ServReq>> reqs;
+    long long next_id = 1;
+    // published by the worker under the lock
+    long long waiting = 0, running = 0, hbm_used = 0;
+    long long n_done = 0, n_failed = 0, n_cancelled = 0, steps = 0;
+    int n_buckets = 0;
+    int worker_rc = RTE_OK;
+    bool stop = false, failed = false;
+    std::thread worker;
+    // worker only
+    std::map<std::string, std::unique_ptr<BucketBase>> buckets;
+    std::list<ServReq *> fifo;
+    long long reserved = 0;  // bytes of allocated slots, idle or busy
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+
+    double now_ms() const
+    {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+static std::string bucket_key(const rte_grid &g, const rte_params &p)
+{
+    char b[512];
+    snprintf(b, sizeof b, "%d,%d,%d,%.17g,%.17g,%.17g,%.17g,%.17g,%.17g|%d,%d,%d,%.17g,%.17g,%d,%d,%d", g.nx, g.ny, g.nz,
+             g.xmin, g.xmax, g.ymin, g.ymax, g.zmin, g.zmax, p.L, p.max_terms, p.stop_rule, p.tol, p.prop_thresh,
+             p.precision, p.exit_quad, p.path_mode);
+    return b;
+}
+
+static BucketBase *make_bucket(rte_server *S, const ServReq *r, int *rc)
+{
+    const rte_grid &gr = r->grid;
+    MomentSet *mom = nullptr;
+    if ((*rc = get_moments(S->ctx, r->params.L, (gr.xmax - gr.xmin) / gr.nx, (gr.ymax - gr.ymin) / gr.ny,
+                           (gr.zmax - gr.zmin) / gr.nz, &mom)))
+        return nullptr;
+    const int cap = S->opts.max_slots_per_bucket;
+    std::unique_ptr<BucketBase> b;
+    switch (r->params.precision) {
+    case RTE_PREC_FP64: b.reset(new Bucket<double, double>(S->ctx, gr, r->params, cap, mom)); break;
+    case RTE_PREC_MIXED: b.reset(new Bucket<float, double>(S->ctx, gr, r->params, cap, mom)); break;
+    default: b.reset(new Bucket<float, float>(S->ctx, gr, r->params, cap, mom)); break;
+    }
+    if (b->init_err) {
+        *rc = fail(b->init_err, "server: resources for a bucket of %dx%dx%d L=%d", gr.nx, gr.ny, gr.nz,
+                   r->params.L);
+        return nullptr;
+    }
+    return b.release();
+}
+
+// Grants slots to waiting requests, first come first served.  A request whose
+// bucket has all its slots busy is passed over; one that needs memory the
+// budget cannot give, even after freeing idle slots, stops the scan.
+static void admit(rte_server *S, std::vector<ServReq *> &finished, std::vector<ServReq *> &admitted)
+{
+    for (auto it = S->fifo.begin(); it != S->fifo.end();) {
+        ServReq *r = *it;
+        if (r->cancel.load()) {
+            r->outcome = REQ_CANCELLED;
+            finished.push_back(r);
+            it = S->fifo.erase(it);
+            continue;
+        }
+        auto bi = S->buckets.find(r->key);
+        if (bi == S->buckets.end()) {
+            int rc = RTE_OK;
+            BucketBase *b = make_bucket(S, r, &rc);
+            if (!b) {
+                r->rc = rc;
+                r->err = g_err;
+                r->outcome = REQ_FAILED;
+                finished.push_back(r);
+                it = S->fifo.erase(it);
+                continue;
+            }
+            bi = S->buckets.emplace(r->key, std::unique_ptr<BucketBase>(b)).first;
+        }
+        BucketBase *B = bi->second.get();
+        if (B->n_busy() >= B->cap) {
+            ++it;
+            continue;
+        }
+        SlotBase *s = B->idle_slot();
+        if (!s) {
+            // a new slot: make room by freeing idle slots of other shapes
+            for (auto &ob : S->buckets) {
+                while (S->reserved + B->slot_bytes > S->opts.hbm_budget && ob.second.get() != B) {
+                    const long long freed = ob.second->evict_one();
+                    if (!freed) break;
+                    S->reserved -= freed;
+                }
+            }
+            if (S->reserved + B->slot_bytes > S->opts.hbm_budget) break;
+            for (auto &ps : B->pool)
+                if (!ps->r && !ps->bytes) {
+                    s = ps.get();
+                    break;
+                }
+            const int rc = B->alloc_slot(*s);  // a slot without memory exists: fewer than cap are busy
+            if (rc) {
+                r->rc = rc;
+                r->err = g_err;
+                r->outcome = REQ_FAILED;
+                finished.push_back(r);
+                it = S->fifo.erase(it);
+                continue;
+            }
+            S->reserved += B->slot_bytes;
+        }
+        s->r = r;
+        B->joining.push_back(s);
+        admitted.push_back(r);
+        it = S->fifo.erase(it);
+    }
+}
+
+static void server_loop(rte_server *S)
+{
+    (void)hipSetDevice(S->ctx->device);
+    std::vector<ServReq *> finished, admitted;
+    for (;;) {
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            if (S->stop) break;
+            while (!S->queue.empty()) {
+                S->fifo.push_back(S->queue.front());
+                S->queue.pop_front();
+            }
+        }
+        admit(S, finished, admitted);
+        const double t_admit = S->now_ms();
+        for (ServReq *r : admitted) r->t_admit = t_admit;
+        int rc = RTE_OK;
+        bool progressed = false, inflight = false;
+        for (auto &kv : S->buckets) {
+            BucketBase *B = kv.second.get();
+            if (!B->step_inflight && (!B->joining.empty() || !B->live.empty())) {
+                if ((rc = B->launch(finished))) break;
+            }
+        }
+        for (auto &kv : S->buckets) {
+            if (rc) break;
+            BucketBase *B = kv.second.get();
+            if (B->step_inflight) {
+                const hipError_t e = hipEventQuery(B->ev_step);
+                if (e == hipSuccess) {
+                    B->complete(finished);
+                    progressed = true;
+                } else if (e != hipErrorNotReady) {
+                    rc = fail(RTE_E_DEVICE, "server: device error (%s)", hipGetErrorString(e));
+                }
+            }
+            if (!rc && B->drain_inflight) {
+                const hipError_t e = hipEventQuery(B->ev_drain);
+                if (e == hipSuccess) {
+                    B->drained(finished);
+                    progressed = true;
+                } else if (e != hipErrorNotReady) {
+                    rc = fail(RTE_E_DEVICE, "server: device error (%s)", hipGetErrorString(e));
+                }
+            }
+            inflight = inflight || B->step_inflight || B->drain_inflight;
+        }
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            const double now = S->now_ms();
+            for (ServReq *r : admitted) {
+                r->state = REQ_RUNNING;
+                ++S->running;
+            }
+            admitted.clear();
+            if (rc) {
+                // a device error ends the service: every request not yet finished fails
+                S->worker_rc = rc;
+                S->failed = true;
+                for (auto &kv : S->reqs)
+                    if (kv.second->state == REQ_RUNNING || kv.second->state == REQ_QUEUED) {
+                        kv.second->rc = rc;
+                        kv.second->err = g_err;
+                        kv.second->outcome = REQ_FAILED;
+                        finished.push_back(kv.second.get());
+                    }
+                S->queue.clear();
+                S->fifo.clear();
+            }
+            for (ServReq *r : finished) {
+                if (r->state == REQ_RUNNING) --S->running;
+                r->state = r->outcome;
+                r->st.latency_ms = now - r->t_submit;
+                r->st.gpu_ms = r->t_admit > 0 ? now - r->t_admit : 0.0;
+                if (r->outcome == REQ_DONE) ++S->n_done;
+                else if (r->outcome == REQ_CANCELLED) ++S->n_cancelled;
+                else ++S->n_failed;
+            }
+            if (progressed) ++S->steps;
+            S->waiting = (long long)S->fifo.size();
+            S->hbm_used = S->reserved;
+            S->n_buckets = (int)S->buckets.size();
+            const bool notify = !finished.empty();
+            finished.clear();
+            if (notify) S->cv_done.notify_all();
+            if (rc) break;
+            if (!progressed) {
+                // sleep until a submission, or poll the device again shortly
+                if (inflight)
+                    S->cv.wait_for(lk, std::chrono::microseconds(50));
+                else if (S->fifo.empty())
+                    S->cv.wait(lk, [&] { return S->stop || !S->queue.empty(); });
+            }
+        }
+    }
+}
+
+extern "C" int rte_server_create(rte_ctx *ctx, const rte_server_opts *opts, rte_server **out)
+{
+    if (!ctx || !out) return fail(RTE_E_ARG, "rte_server_create: NULL argument");
+    *out = nullptr;
+    auto S = std::make_unique<rte_server>();
+    S->ctx = ctx;
+    if (opts) S->opts = *opts;
+    if (S->opts.hbm_budget <= 0) {
+        size_t fr = 0, tot = 0;
+        HIP_TRY(hipSetDevice(ctx->device));
+        HIP_TRY(hipMemGetInfo(&fr, &tot));
+        S->opts.hbm_budget = (long long)(0.8 * (double)fr);
+    }
+    if (S->opts.max_slots_per_bucket <= 0) S->opts.max_slots_per_bucket = 64;
+    if (S->opts.max_queue <= 0) S->opts.max_queue = 100000;
+    rte_server *raw = S.release();
+    raw->worker = std::thread(server_loop, raw);
+    *out = raw;
+    return RTE_OK;
+}
+
+extern "C" int rte_server_destroy(rte_server *S)
+{
+    if (!S) return RTE_OK;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+    }
+    S->cv.notify_all();
+    if (S->worker.joinable()) S->worker.join();
+    S->buckets.clear();  // waits for each bucket's stream, then frees its slots
+    delete S;
+    return RTE_OK;
+}
+
+static int beam_columns(const rte_grid *grid, const rte_beam *beam)
+{
+    const double dx = (grid->xmax - grid->xmin) / grid->nx, dy = (grid->ymax - grid->ymin) / grid->ny;
+    const double half = (beam->diam + 1.0e-16) / 2.0;
+    int cols = 0;
+    for (int j = 0; j < grid->ny; ++j)
+        for (int i = 0; i < grid->nx; ++i) {
+            const double xc = (double)i * dx + grid->xmin + dx / 2.0, yc = (double)j * dy + grid->ymin + dy / 2.0;
+            if (std::sqrt((xc - beam->pos_x) * (xc - beam->pos_x) + (yc - beam->pos_y) * (yc - beam->pos_y)) <= half)
+                ++cols;
+        }
+    return cols;
+}
+
+extern "C" int rte_submit(rte_server *S, const rte_grid *grid, const rte_optics *optics, const rte_beam *beam,
+                          const rte_params *params, long long *id)
+{
+    if (!S || !grid || !optics || !beam || !params || !id) return fail(RTE_E_ARG, "rte_submit: NULL argument");
+    int rc;
+    if ((rc = check_grid(grid)) || (rc = check_optics(grid, optics)) || (rc = check_params(params))) return rc;
+    if (!std::isfinite(beam->pos_x) || !std::isfinite(beam->pos_y) || !(beam->diam >= 0.0) ||
+        !std::isfinite(beam->mag))
+        return fail(RTE_E_ARG, "rte_submit: invalid beam");
+    auto r = std::make_unique<ServReq>();
+    if (!(r->beam_cols = beam_columns(grid, beam))) return fail(RTE_E_ARG, "rte_submit: beam hits no voxel column");
+    r->grid = *grid;
+    const size_t N = (size_t)grid->nx * grid->ny * grid->nz;
+    r->mua.assign(optics->mu_a, optics->mu_a + N);
+    r->mus.assign(optics->mu_s, optics->mu_s + N);
+    r->mutd.resize(N);
+    r->mutf.resize(N);
+    r->musf.resize(N);
+    for (size_t v = 0; v < N; ++v) {
+        r->mutd[v] = r->mua[v] + r->mus[v];
+        r->mutf[v] = (float)r->mutd[v];
+        r->musf[v] = (float)r->mus[v];
+    }
+    r->g = optics->g;
+    r->beam = *beam;
+    r->params = *params;
+    r->key = bucket_key(*grid, *params);
+    r->footprint = rte_solve_footprint(grid, params, params->path_mode);
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->failed) return fail(RTE_E_DEVICE, "rte_submit: the server stopped after a device error");
+    if (r->footprint > S->opts.hbm_budget)
+        return fail(RTE_E_NOMEM, "rte_submit: a solve of this shape needs %lld bytes, the budget is %lld",
+                    r->footprint, S->opts.hbm_budget);
+    if ((long long)S->queue.size() + S->waiting >= S->opts.max_queue)
+        return fail(RTE_E_BUSY, "rte_submit: queue full");
+    r->id = S->next_id++;
+    r->t_submit = S->now_ms();
+    *id = r->id;
+    S->queue.push_back(r.get());
+    S->reqs.emplace(r->id, std::move(r));
+    S->cv.notify_all();
+    return RTE_OK;
+}
+
+static int poll_locked(rte_server *S, long long id, double *exitance, rte_stats *stats, int *status)
+{
+    auto it = S->reqs.find(id);
+    if (it == S->reqs.end()) return fail(RTE_E_NOTFOUND, "request %lld unknown or already collected", id);
+    ServReq *r = it->second.get();
+    if (status) *status = r->state;
+    if (r->state == REQ_DONE) {
+        if (exitance) std::memcpy(exitance, r->img.data(), sizeof(double) * r->img.size());
+        if (stats) *stats = r->st;
+        S->reqs.erase(it);
+        return RTE_OK;
+    }
+    if (r->state == REQ_FAILED) {
+        const int rc = r->rc ? r->rc : RTE_E_DEVICE;
+        g_err = r->err;
+        S->reqs.erase(it);
+        return rc;
+    }
+    if (r->state == REQ_CANCELLED) S->reqs.erase(it);
+    return RTE_OK;
+}
+
+extern "C" int rte_poll(rte_server *S, long long id, double *exitance, rte_stats *stats, int *status)
+{
+    if (!S) return fail(RTE_E_ARG, "rte_poll: NULL server");
+    std::lock_guard<std::mutex> lk(S->mu);
+    return poll_locked(S, id, exitance, stats, status);
+}
+
+extern "C" int rte_wait(rte_server *S, long long id, double timeout_ms, double *exitance, rte_stats *stats,
+                        int *status)
+{
+    if (!S) return fail(RTE_E_ARG, "rte_wait: NULL server");
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto ready = [&] {
+        auto it = S->reqs.find(id);
+        return it == S->reqs.end() || it->second->state >= REQ_DONE;
+    };
+    if (timeout_ms < 0) {
+        S->cv_done.wait(lk, ready);
+    } else if (!S->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), ready)) {
+        if (status) *status = S->reqs[id]->state;
+        return fail(RTE_E_BUSY, "rte_wait: request %lld not finished after %.0f ms", id, timeout_ms);
+    }
+    return poll_locked(S, id, exitance, stats, status);
+}
+
+extern "C" int rte_cancel(rte_server *S, long long id)
+{
+    if (!S) return fail(RTE_E_ARG, "rte_cancel: NULL server");
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->reqs.find(id);
+    if (it == S->reqs.end()) return fail(RTE_E_NOTFOUND, "request %lld unknown or already collected", id);
+    it->second->cancel.store(true);
+    S->cv.notify_all();
+    return RTE_OK;
+}
+
+extern "C" int rte_server_info(rte_server *S, rte_server_stats *out)
+{
+    if (!S || !out) return fail(RTE_E_ARG, "rte_server_info: NULL argument");
+    std::lock_guard<std::mutex> lk(S->mu);
+    std::memset(out, 0, sizeof *out);
+    out->queued = (long long)S->queue.size() + S->waiting;
+    out->running = S->running;
+    out->done = S->n_done;
+    out->failed = S->n_failed;
+    out->cancelled = S->n_cancelled;
+    out->steps = S->steps;
+    out->hbm_used = S->hbm_used;
+    out->hbm_budget = S->opts.hbm_budget;
+    out->buckets = S->n_buckets;
+    out->worker_status = S->worker_rc;
+    return RTE_OK;
 }

@@ -101,3 +101,37 @@ def config5_beams(seed=5, n=32, per_medium=5, dx=0.1):
     xs = voxel_centres(n, dx)
     idx = np.floor(st.uniform(2 * per_medium) * n).astype(int)
     return [(xs[idx[2 * b]], xs[idx[2 * b + 1]], dx, 1.0) for b in range(per_medium)]
+
+
+def serving_requests(n_req=2000, seed=4, edges=(8, 12, 16, 24, 32), orders=(3, 5), rate=200.0, dx=0.1,
+                     tol=1e-6, max_terms=30, precision=None):
+    """Mixed serving stream: Poisson arrivals at `rate` per second (all at t=0
+    if rate is 0), cubic grid edge uniform over `edges`, L over `orders`,
+    g ~ U[0.7, 0.9], background mu_s ~ U[0.5, 1.5] and mu_a ~ U[0.005, 0.05]
+    per cm, each with a lognormal per-voxel factor (sigma 0.2), and a pencil
+    beam one voxel wide on a random top-face voxel column.  Solves run in
+    mixed precision to a 1e-6 relative change of total exitance (cap 30)."""
+    from . import rte
+    prec = rte.PREC_MIXED if precision is None else precision
+    st = Stream(seed * 104729 + 17)
+    t, out = 0.0, []
+    for k in range(n_req):
+        u = st.uniform(8)
+        if rate > 0:
+            t += -np.log1p(-u[0]) / rate
+        n = int(edges[min(int(u[1] * len(edges)), len(edges) - 1)])
+        L = int(orders[min(int(u[2] * len(orders)), len(orders) - 1)])
+        g = 0.7 + 0.2 * u[3]
+        mus = 0.5 + 1.0 * u[4]
+        mua = 0.005 + 0.045 * u[5]
+        N = n ** 3
+        ma = mua * np.exp(0.2 * st.normal(N))
+        ms = mus * np.exp(0.2 * st.normal(N))
+        xs = voxel_centres(n, dx)
+        bx = xs[min(int(u[6] * n), n - 1)]
+        by = xs[min(int(u[7] * n), n - 1)]
+        out.append(dict(t=t, shape=(n, n, n), bounds=cube_bounds(n, dx), mu_a=ma, mu_s=ms, g=g,
+                        beam=(bx, by, dx, 1.0), L=L,
+                        params=rte.make_params(L, precision=prec, stop_rule=rte.STOP_DELTA, tol=tol,
+                                               max_terms=max_terms)))
+    return out

@@ -68,6 +68,20 @@ class Prof(C.Structure):
                 ("L", C.c_int), ("precision", C.c_int)]
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [("hbm_budget", C.c_longlong), ("max_slots_per_bucket", C.c_int), ("max_queue", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("queued", C.c_longlong), ("running", C.c_longlong), ("done", C.c_longlong),
+                ("failed", C.c_longlong), ("cancelled", C.c_longlong), ("steps", C.c_longlong),
+                ("hbm_used", C.c_longlong), ("hbm_budget", C.c_longlong), ("buckets", C.c_int),
+                ("worker_status", C.c_int)]
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_CANCELLED = 0, 1, 2, 3, 4
+
+
 # every entry point of include/rte.h: name -> (restype, argtypes)
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -94,6 +108,14 @@ SIGNATURES = {
                                _dp, _dp, C.POINTER(Stats)]),
     "rte_fisher": (C.c_int, [_vp, _vp, C.POINTER(Beam), C.POINTER(Params), C.POINTER(GradOpts), _ip, C.c_int,
                              C.c_int, _dp, _dp, _dp, _dp, C.POINTER(Stats)]),
+    "rte_server_create": (C.c_int, [_vp, C.POINTER(ServerOpts), C.POINTER(_vp)]),
+    "rte_server_destroy": (C.c_int, [_vp]),
+    "rte_submit": (C.c_int, [_vp, C.POINTER(Grid), C.POINTER(Optics), C.POINTER(Beam), C.POINTER(Params),
+                             C.POINTER(C.c_longlong)]),
+    "rte_poll": (C.c_int, [_vp, C.c_longlong, _dp, C.POINTER(Stats), _ip]),
+    "rte_wait": (C.c_int, [_vp, C.c_longlong, C.c_double, _dp, C.POINTER(Stats), _ip]),
+    "rte_cancel": (C.c_int, [_vp, C.c_longlong]),
+    "rte_server_info": (C.c_int, [_vp, C.POINTER(ServerStats)]),
     "rte_propagate": (C.c_int, [_vp, C.POINTER(Grid), C.POINTER(Optics), C.c_int, C.c_double,
                                 C.c_int, C.c_int, _dp, _dp]),
     "rte_scatter": (C.c_int, [_vp, C.POINTER(Grid), C.POINTER(Optics), C.c_int, _dp]),
@@ -177,10 +199,13 @@ class Context:
     def __init__(self, device=0):
         self.h = _vp()
         self._media = weakref.WeakSet()
+        self._servers = weakref.WeakSet()
         _check(lib().rte_ctx_create(device, C.byref(self.h)))
 
     def close(self):
         if self.h:
+            for s in list(self._servers):  # a server's worker uses the context
+                s.close()
             for m in list(self._media):   # media die with their context
                 m.close()
             lib().rte_ctx_destroy(self.h)
@@ -319,3 +344,84 @@ class _Maps:
 
 def footprint(shape, bounds, params, path_mode=PATH_CACHED):
     return lib().rte_solve_footprint(C.byref(Grid(*shape, *bounds)), C.byref(params), path_mode)
+
+
+def _stats_dict(s):
+    return dict(terms=s.terms, lam=s.lambda_last, rel=s.rel_change_last, total=s.total_exitance,
+                ballistic=s.ballistic_total, gpu_ms=s.gpu_ms, latency_ms=s.latency_ms)
+
+
+class Server:
+    """Continuous-batching solve service on one context (rte_server_*).
+
+    While it exists its worker thread owns the context: make no other calls on
+    that context until close()."""
+
+    def __init__(self, ctx, hbm_budget=0, max_slots_per_bucket=0, max_queue=0):
+        self.ctx = ctx
+        self.h = _vp()
+        self._shapes = {}
+        _check(lib().rte_server_create(ctx.h, C.byref(ServerOpts(hbm_budget, max_slots_per_bucket, max_queue)),
+                                       C.byref(self.h)))
+        ctx._servers.add(self)
+
+    def submit(self, shape, bounds, mu_a, mu_s, g, beam, params, n=1.0, c=2.9979e10):
+        nx, ny, nz = shape
+        N = nx * ny * nz
+        ma = _f64(np.broadcast_to(mu_a, (N,)) if np.ndim(mu_a) == 0 else np.ravel(mu_a))
+        ms = _f64(np.broadcast_to(mu_s, (N,)) if np.ndim(mu_s) == 0 else np.ravel(mu_s))
+        rid = C.c_longlong(0)
+        _check(lib().rte_submit(self.h, C.byref(Grid(nx, ny, nz, *bounds)), C.byref(Optics(_p(ma), _p(ms), g, n, c)),
+                                C.byref(Beam(*beam)), C.byref(params), C.byref(rid)))
+        self._shapes[rid.value] = (ny, nx)
+        return rid.value
+
+    def _collect(self, rid, fn, *extra):
+        ny, nx = self._shapes[rid]
+        img = np.zeros(nx * ny)
+        st = Stats()
+        status = C.c_int(0)
+        try:
+            _check(fn(self.h, rid, *extra, _p(img), C.byref(st), C.byref(status)))
+        except RteError as e:
+            if e.code != E_BUSY:
+                self._shapes.pop(rid, None)
+            raise
+        if status.value in (REQ_DONE, REQ_CANCELLED):
+            del self._shapes[rid]
+        if status.value == REQ_DONE:
+            return status.value, img.reshape(ny, nx), _stats_dict(st)
+        return status.value, None, None
+
+    def poll(self, rid):
+        """(status, image or None, stats or None); raises for a failed solve."""
+        return self._collect(rid, lib().rte_poll)
+
+    def wait(self, rid, timeout_ms=-1.0):
+        """As poll, blocking until the solve leaves the service (or timeout_ms)."""
+        return self._collect(rid, lib().rte_wait, timeout_ms)
+
+    def cancel(self, rid):
+        _check(lib().rte_cancel(self.h, rid))
+
+    def info(self):
+        s = ServerStats()
+        _check(lib().rte_server_info(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.h:
+            lib().rte_server_destroy(self.h)
+            self.h = _vp()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

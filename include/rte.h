@@ -129,7 +129,7 @@ typedef struct rte_medium rte_medium;
 int rte_ctx_create(int device, rte_ctx **out);
 /* also destroys the media still attached to the context (their handles become invalid) */
 int rte_ctx_destroy(rte_ctx *ctx);
-/* at most max_rhs (1..5, default 5) beams on one medium share a propagation
+/* at most max_rhs (1..5, default 2) beams on one medium share a propagation
  * launch on the mixed-precision cached path; the result bits do not depend on it */
 int rte_ctx_set_group_size(rte_ctx *ctx, int max_rhs);
 /* on != 0: time every launch from now on (resets the counters); 0: stop */
@@ -197,6 +197,49 @@ int rte_gradient(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte_pa
 int rte_fisher(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte_params *params,
                const rte_grad_opts *opts, const int *partition, int nparam, int wrt, double *F,
                double *chol, double *J, double *exitance, rte_stats *stats);
+
+/* ---- serving --------------------------------------------------------------
+ * Continuous batching (no counterpart in the reference, which runs one solve
+ * per process, GPU package/src/rte.cu:89-227).  A worker thread owns the
+ * context while the server exists (make no other call on it until
+ * rte_server_destroy): requests are granted a solve slot of the bucket of
+ * their shape (grid, L, precision, path, stop rule) first come first served
+ * while the slots' device memory fits hbm_budget, and advance one Neumann
+ * term per step of their bucket with every other resident solve of it;
+ * buckets step independently and overlap on the device.  Finished solves
+ * leave their slot to the next request of the shape (idle slots are freed
+ * when another shape needs the memory).  A solve's result is bit-identical to
+ * rte_solve_media's for it, whatever it was batched with. */
+typedef struct rte_server rte_server;
+typedef struct {
+    long long hbm_budget;     /* bytes for resident solves; <= 0: 80 % of free device memory */
+    int max_slots_per_bucket; /* resident solves per shape bucket; <= 0: 64 */
+    int max_queue;            /* queued requests before rte_submit returns RTE_E_BUSY; <= 0: 100000 */
+} rte_server_opts;
+typedef struct {
+    long long queued, running, done, failed, cancelled, steps;
+    long long hbm_used; /* device memory of the solve slots, busy and pooled idle ones */
+    long long hbm_budget;
+    int buckets;
+    int worker_status; /* last device error of the worker (RTE_OK if none) */
+} rte_server_stats;
+
+int rte_server_create(rte_ctx *ctx, const rte_server_opts *opts, rte_server **out);
+int rte_server_destroy(rte_server *s);
+/* validates and queues a solve (the maps are copied); RTE_E_BUSY if the queue is
+ * full, RTE_E_NOMEM if the solve can never fit the budget */
+int rte_submit(rte_server *s, const rte_grid *grid, const rte_optics *optics, const rte_beam *beam,
+               const rte_params *params, long long *id);
+/* *status: 0 queued, 1 running, 2 done (exitance/stats written, id released),
+ * 3 failed (the solve's error code is returned), 4 cancelled; RTE_E_NOTFOUND for an
+ * unknown or released id.  stats->latency_ms is submit-to-result, gpu_ms admit-to-result. */
+int rte_poll(rte_server *s, long long id, double *exitance, rte_stats *stats, int *status);
+/* as rte_poll, blocking until the request leaves the queue/device or timeout_ms passes
+ * (< 0: no limit; RTE_E_BUSY on timeout) */
+int rte_wait(rte_server *s, long long id, double timeout_ms, double *exitance, rte_stats *stats,
+             int *status);
+int rte_cancel(rte_server *s, long long id);
+int rte_server_info(rte_server *s, rte_server_stats *out);
 
 /* ---- operators (for operator-level parity) ------------------------------ */
 /* A: propagation incl. self-voxel block and dV (CPU package/src/Kernels_selfsub.c:450-538,

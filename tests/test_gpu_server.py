@@ -1,0 +1,149 @@
+"""Continuous-batching service (rte_server_*): every result is bit-identical to
+the same solve run alone through rte_solve_media, whatever it was batched with
+and whenever it arrived; admission keeps the resident footprint within the HBM
+budget; cancellation and back-pressure behave as declared."""
+import time
+
+import numpy as np
+import pytest
+
+from doi_amd import media, rte
+from tests.helpers import bounds_for, hetero_maps
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests():
+    """Mixed shapes, orders and precisions; two beams share one medium."""
+    out = []
+    for i, (shape, L, prec, g) in enumerate([
+            ((8, 8, 8), 3, rte.PREC_MIXED, 0.8),
+            ((12, 12, 12), 5, rte.PREC_MIXED, 0.9),
+            ((8, 8, 8), 3, rte.PREC_MIXED, 0.7),
+            ((6, 7, 5), 2, rte.PREC_FP64, 0.8),
+            ((12, 12, 12), 5, rte.PREC_MIXED, 0.9),
+            ((8, 8, 8), 3, rte.PREC_FP32, 0.8),
+            ((16, 16, 16), 3, rte.PREC_MIXED, 0.85)]):
+        ma, ms = hetero_maps(shape, seed=40 + i, sigma=0.2)
+        beam = (0.05 * (i % 3) - 0.05, 0.05 * (i % 2), 0.2, 1.0)
+        rule = rte.STOP_LAMBDA if i % 2 else rte.STOP_DELTA
+        p = rte.make_params(L, precision=prec, stop_rule=rule, tol=1e-6, max_terms=30)
+        out.append(dict(shape=shape, bounds=bounds_for(shape), ma=ma, ms=ms, g=g, beam=beam, p=p))
+    # same medium as request 1, another beam: grouped with it while both are resident
+    r1 = dict(out[1])
+    r1["beam"] = (0.15, -0.05, 0.1, 1.0)
+    out.append(r1)
+    return out
+
+
+def _alone(ctx, r):
+    img, st = ctx.solve(r["shape"], r["bounds"], r["ma"], r["ms"], r["g"], r["beam"], r["p"])
+    return img, st
+
+
+def _submit(srv, r):
+    return srv.submit(r["shape"], r["bounds"], r["ma"], r["ms"], r["g"], r["beam"], r["p"])
+
+
+def test_results_identical_alone_and_in_the_mix(ctx):
+    reqs = _requests()
+    ref = [_alone(ctx, r) for r in reqs]
+    sctx = rte.Context(0)
+    try:
+        # each request alone through the service
+        with rte.Server(sctx) as srv:
+            solo = []
+            for r in reqs:
+                rid = _submit(srv, r)
+                solo.append(srv.wait(rid, 60000.0))
+        # all of them staggered: later ones arrive while earlier ones are resident
+        with rte.Server(sctx) as srv:
+            ids = []
+            for k, r in enumerate(reqs):
+                ids.append(_submit(srv, r))
+                if k % 3 == 2:
+                    time.sleep(0.003)
+            mixed = [srv.wait(i, 60000.0) for i in ids]
+            info = srv.info()
+        assert info["done"] == len(reqs) and info["failed"] == 0 and info["running"] == 0
+        assert info["buckets"] >= 5 and info["worker_status"] == 0
+    finally:
+        sctx.close()
+    for k, ((img, st), s1, s2) in enumerate(zip(ref, solo, mixed)):
+        assert s1[0] == rte.REQ_DONE and s2[0] == rte.REQ_DONE, k
+        assert s1[2]["terms"] == st["terms"] and s2[2]["terms"] == st["terms"], k
+        assert np.array_equal(s1[1], img), (k, np.max(np.abs(s1[1] - img)))
+        assert np.array_equal(s2[1], img), (k, np.max(np.abs(s2[1] - img)))
+        assert s2[2]["latency_ms"] >= s2[2]["gpu_ms"] >= 0.0
+
+
+def test_admission_budget_cancel_and_back_pressure():
+    shape = (12, 12, 12)
+    ma, ms = hetero_maps(shape, seed=3, sigma=0.2)
+    p = rte.make_params(5, precision=rte.PREC_MIXED, stop_rule=rte.STOP_DELTA, tol=1e-6, max_terms=30)
+    g = rte.Grid(*shape, *bounds_for(shape))
+    fp = rte.lib().rte_solve_footprint(g, p, p.path_mode)
+    assert fp > 0
+    sctx = rte.Context(0)
+    try:
+        # room for two resident solves at a time
+        with rte.Server(sctx, hbm_budget=2 * fp + fp // 2, max_queue=4) as srv:
+            ids, busy = [], 0
+            for k in range(12):
+                try:
+                    ids.append(srv.submit(shape, bounds_for(shape), ma, ms, 0.8, (0.0, 0.0, 0.2, 1.0), p))
+                except rte.RteError as e:
+                    assert e.code == rte.E_BUSY
+                    busy += 1
+            assert busy > 0
+            peak = 0
+            srv.cancel(ids[-1])
+            out = []
+            for rid in ids:
+                peak = max(peak, srv.info()["hbm_used"])
+                out.append(srv.wait(rid, 60000.0))
+            info = srv.info()
+            # too large for the budget: refused at submission
+            big = (24, 24, 24)
+            with pytest.raises(rte.RteError) as ei:
+                srv.submit(big, bounds_for(big), 0.01, 1.0, 0.8, (0.0, 0.0, 0.2, 1.0), p)
+            assert ei.value.code == rte.E_NOMEM
+            with pytest.raises(rte.RteError) as ei:
+                srv.cancel(123456789)
+            assert ei.value.code == rte.E_NOTFOUND
+        # pooled slots stay allocated, within the budget
+        assert peak <= 2 * fp + fp // 2 and info["hbm_used"] <= 2 * fp + fp // 2
+        assert info["running"] == 0 and info["queued"] == 0
+        status = [o[0] for o in out]
+        assert status[-1] in (rte.REQ_CANCELLED, rte.REQ_DONE)
+        done = [o for o in out if o[0] == rte.REQ_DONE]
+        assert len(done) >= len(ids) - 1
+        for o in done[1:]:
+            assert np.array_equal(o[1], done[0][1])
+        # invalid optics are rejected before any device work
+        with rte.Server(sctx) as srv:
+            bad = ma.copy()
+            bad[5] = np.nan
+            with pytest.raises(rte.RteError) as ei:
+                srv.submit(shape, bounds_for(shape), bad, ms, 0.8, (0.0, 0.0, 0.2, 1.0), p)
+            assert ei.value.code == rte.E_ARG
+    finally:
+        sctx.close()
+
+
+def test_served_config4_sample_matches_alone(ctx):
+    """A small seeded sample of the mixed serving stream (grid edges 8-32,
+    L 3/5, random g, mu and beam): served results equal the solves run alone."""
+    reqs = media.serving_requests(12, seed=4, edges=(8, 12, 16), rate=0.0)
+    sctx = rte.Context(0)
+    try:
+        with rte.Server(sctx) as srv:
+            ids = [srv.submit(r["shape"], r["bounds"], r["mu_a"], r["mu_s"], r["g"], r["beam"], r["params"])
+                   for r in reqs]
+            served = [srv.wait(i, 60000.0) for i in ids]
+    finally:
+        sctx.close()
+    for r, s in zip(reqs, served):
+        img, st = ctx.solve(r["shape"], r["bounds"], r["mu_a"], r["mu_s"], r["g"], r["beam"], r["params"])
+        assert s[0] == rte.REQ_DONE and s[2]["terms"] == st["terms"]
+        assert np.array_equal(s[1], img)
