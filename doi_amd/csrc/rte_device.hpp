@@ -38,9 +38,11 @@ __device__ __forceinline__ double big_value<double>() { return 1.0e300; }
 // z, then y, then x; the destination contributes the half step 0.5/n of the
 // axis it was entered through (GPU package/src/kernels.cu:49-145).
 // ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ T march_tau(const __attribute__((address_space(1))) T *mut, int v, int ix, int iy, int iz,
-                                       int nx, int nxy, T r)
+// The next voxel's mu_t is loaded one step ahead of its use (the voxel
+// sequence does not depend on the loaded values), so the load latency
+// overlaps the crossing arithmetic.  P: a global or LDS pointer.
+template <typename T, typename P>
+__device__ __forceinline__ T march_tau(P mut, int v, int ix, int iy, int iz, int nx, int nxy, T r)
 {
     const int n0 = abs(ix), n1 = abs(iy), n2 = abs(iz);
     const T BIG = big_value<T>();
@@ -51,19 +53,22 @@ __device__ __forceinline__ T march_tau(const __attribute__((address_space(1))) T
     const int st0 = ix > 0 ? 1 : -1, st1 = iy > 0 ? nx : -nx, st2 = iz > 0 ? nxy : -nxy;
     const int nsteps = n0 + n1 + n2;
     T prev = T(0), acc = T(0), last = T(0);
+    T mu = mut[v];
     for (int t = 0; t < nsteps; ++t) {
         const bool pz = (a2 <= a0) && (a2 <= a1);
         const bool py = !pz && (a1 <= a0);
         const T a = pz ? a2 : (py ? a1 : a0);
-        acc += mut[v] * (a - prev);
-        prev = a;
         v += pz ? st2 : (py ? st1 : st0);
+        const T mu_next = mut[v];
+        acc += mu * (a - prev);
+        prev = a;
         last = pz ? inc2 : (py ? inc1 : inc0);
         a2 = pz ? a2 + inc2 : a2;
         a1 = py ? a1 + inc1 : a1;
         a0 = (!pz && !py) ? a0 + inc0 : a0;
+        mu = mu_next;
     }
-    acc += mut[v] * T(0.5) * last;
+    acc += mu * T(0.5) * last;
     return acc * r;
 }
 

@@ -616,16 +616,14 @@ static int ensure_ktab(rte_medium *m, double thr, hipStream_t on = nullptr)
     if (tab.p && tthr == thr) return RTE_OK;
     const GridDev<T> g = make_grid<T>(m->grid);
     HIP_TRY(tab.alloc((size_t)g.N * g.npad * sizeof(T)));
-    const int nchunk = pick_chunks(g.nblk, 1, g.N);
-    const int clen = (g.N + nchunk - 1) / nchunk;
     const T *mut = sizeof(T) == 4 ? (const T *)m->mut_f.p : (const T *)m->mut_d.p;
     const T thr2 = thr > 0 ? T(thr * thr) : T(sizeof(T) == 4 ? 1e30 : 1e300);
     if (st == m->ctx->stream) {
         LaunchTimer tm(m->ctx, 1);
-        HIP_TRY(launch_ktab<T>(g, (g.N + clen - 1) / clen, clen, thr2, mut, tab.as<T>(), st));
+        HIP_TRY(launch_ktab<T>(g, thr2, mut, tab.as<T>(), st));
         tm.done((double)g.N * g.N);
     } else {
-        HIP_TRY(launch_ktab<T>(g, (g.N + clen - 1) / clen, clen, thr2, mut, tab.as<T>(), st));
+        HIP_TRY(launch_ktab<T>(g, thr2, mut, tab.as<T>(), st));
     }
     tthr = thr;
     return RTE_OK;
@@ -1919,7 +1917,7 @@ struct Bucket : BucketBase {
         HIP_TRY(hipMemcpyAsync(s.musf.p, r->musf.data(), n * 4, hipMemcpyHostToDevice, stream));
         const T *mut = sizeof(T) == 4 ? (const T *)s.mutf.p : (const T *)s.mutd.p;
         if (p.path_mode == RTE_PATH_CACHED)
-            HIP_TRY(launch_ktab<T>(g, nch, clen, pa.thr2, mut, s.ktab.template as<T>(), stream));
+            HIP_TRY(launch_ktab<T>(g, pa.thr2, mut, s.ktab.template as<T>(), stream));
         const size_t F = (size_t)N * K;
         HIP_TRY(launch_fill<T>(s.W.template as<T>(), F, T(0), stream));
         SolveDev<T, TA> &v = sv[s.idx];

@@ -98,8 +98,7 @@ template <typename T, typename TA>
 hipError_t launch_prop(int L, int path, int nr, const PropArgs<T> &A, const SolveDev<T, TA> *sv, int ngroups,
                        hipStream_t st);
 template <typename T>
-hipError_t launch_ktab(const GridDev<T> &g, int nchunk, int chunk_len, T thr2, const T *mut, T *ktab,
-                       hipStream_t st);
+hipError_t launch_ktab(const GridDev<T> &g, T thr2, const T *mut, T *ktab, hipStream_t st);
 template <typename T, typename TA>
 hipError_t launch_combine(int L, const CombArgs<T, TA> &A, const SolveDev<T, TA> *sv, int nsolve,
                           hipStream_t st);

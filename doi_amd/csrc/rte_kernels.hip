@@ -128,36 +128,80 @@ __global__ __launch_bounds__(kWave) void k_prop(PropArgs<T> A, const SolveDev<T,
 
 // ---------------------------------------------------------------------------
 // Per-medium table kf(d, s) = e^{-tau(d,s)}/r^2 (0 for d == s, r >= thresh and
-// padding lanes), layout [s][slot].  grid = (nblk, nchunk), block = 64.
+// padding), layout [s][slot(d)].  The ray between two voxel centres is the
+// same both ways, so one wave marches the pairs of a brick pair (bd, bs) with
+// bs <= bd once (from s to d) and writes both entries: the direct one
+// (row s, the wave's 64 destination slots: one coalesced row segment per
+// source) and, off the diagonal, the mirrored one (row d, the source brick's
+// slots), staged one x row of 4 sources at a time in registers and written
+// as a 16-byte run per lane.  mu_t is read from LDS when the whole map fits.
+// grid = ceil(ntiles / waves per block), block = KT threads.
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(kWave) void k_ktab(GridDev<T> g, int chunk_len, T thr2,
-                                               const T *__restrict__ mut, T *__restrict__ ktab)
+template <typename T, bool LDS>
+__global__ __launch_bounds__(1024) void k_ktab(GridDev<T> g, int ntiles, T thr2, const T *__restrict__ mut,
+                                               T *__restrict__ ktab)
 {
-    const int lane = threadIdx.x, blk = blockIdx.x, chunk = blockIdx.y;
+    extern __shared__ __align__(16) unsigned char ktab_lds[];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1);
+    if constexpr (LDS) {
+        T *sm = reinterpret_cast<T *>(ktab_lds);
+        for (int i = tid; i < g.N; i += blockDim.x) sm[i] = mut[i];
+        __syncthreads();
+    }
+    const int tile = blockIdx.x * (blockDim.x / kWave) + tid / kWave;
+    if (tile >= ntiles) return;
+    // tile -> (bd, bs), bs <= bd
+    int bd = (int)((sqrtf(8.0f * (float)tile + 1.0f) - 1.0f) * 0.5f);
+    while (bd > 0 && bd * (bd + 1) / 2 > tile) --bd;
+    while ((bd + 1) * (bd + 2) / 2 <= tile) ++bd;
+    const int bs = tile - bd * (bd + 1) / 2;
     int di, dj, dk;
-    lane_dest(g, blk, lane, di, dj, dk);
-    const bool valid = di < g.nx && dj < g.ny && dk < g.nz;
+    lane_dest(g, bd, lane, di, dj, dk);
+    const bool dvalid = di < g.nx && dj < g.ny && dk < g.nz;
     const int cdi = min(di, g.nx - 1), cdj = min(dj, g.ny - 1), cdk = min(dk, g.nz - 1);
-    const int slot = blk * kWave + lane;
-    const int s0 = chunk * chunk_len;
-    const int s1 = min(g.N, s0 + chunk_len);
     const int nxy = g.nx * g.ny;
-    int si = s0 % g.nx, sj = (s0 / g.nx) % g.ny, sk = s0 / nxy;
-    for (int s = s0; s < s1; ++s) {
-        const int ix = cdi - si, iy = cdj - sj, iz = cdk - sk;
-        const T ddx = T(ix) * g.dx, ddy = T(iy) * g.dy, ddz = T(iz) * g.dz;
-        const T r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-        const bool self = (ix == 0) && (iy == 0) && (iz == 0);
-        const bool use = valid && !self && (r2 < thr2);
-        const T r = sqrt(self ? T(1) : r2);
-        const T tau = use ? march_tau<T>(as_global<T>(mut), s, ix, iy, iz, g.nx, nxy, r) : T(0);
-        ktab[(size_t)s * g.npad + slot] = use ? dev_exp<T>(-tau) / r2 : T(0);
-        if (++si == g.nx) {
-            si = 0;
-            if (++sj == g.ny) {
-                sj = 0;
-                ++sk;
+    const int dslot = bd * kWave + lane;
+    const int dlin = cdi + g.nx * cdj + nxy * cdk;
+    const int bxi = bs % g.bx, byi = (bs / g.bx) % g.by, bzi = bs / (g.bx * g.by);
+    const bool mirror = bs != bd;
+    for (int q = 0; q < 16; ++q) {  // the source brick's x rows, 4 voxels each
+        T m[4];
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+            const int si = bxi * kBrick + jj, sj = byi * kBrick + (q & 3), sk = bzi * kBrick + (q >> 2);
+            T kf = T(0);
+            if (si < g.nx && sj < g.ny && sk < g.nz) {  // wave-uniform
+                const int ix = cdi - si, iy = cdj - sj, iz = cdk - sk;
+                const T ddx = T(ix) * g.dx, ddy = T(iy) * g.dy, ddz = T(iz) * g.dz;
+                const T r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                const bool self = (ix == 0) && (iy == 0) && (iz == 0);
+                const bool use = dvalid && !self && (r2 < thr2);
+                const T r = sqrt(self ? T(1) : r2);
+                const int slin = si + g.nx * sj + nxy * sk;
+                T tau = T(0);
+                if (use) {
+                    if constexpr (LDS)
+                        tau = march_tau<T>((const __attribute__((address_space(3))) T *)ktab_lds, slin, ix, iy, iz,
+                                           g.nx, nxy, r);
+                    else
+                        tau = march_tau<T>(as_global<T>(mut), slin, ix, iy, iz, g.nx, nxy, r);
+                }
+                kf = use ? dev_exp<T>(-tau) / r2 : T(0);
+                ktab[(size_t)slin * g.npad + dslot] = kf;
+            }
+            m[jj] = kf;
+        }
+        if (mirror && dvalid) {
+            using V = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+            constexpr int per = sizeof(V) / sizeof(T);
+            V *row = reinterpret_cast<V *>(ktab + (size_t)dlin * g.npad + bs * kWave + q * 4);
+#pragma unroll
+            for (int u = 0; u < 4 / per; ++u) {
+                V x;
+                T *xs = reinterpret_cast<T *>(&x);
+#pragma unroll
+                for (int e = 0; e < per; ++e) xs[e] = m[u * per + e];
+                row[u] = x;
             }
         }
     }
@@ -528,11 +572,32 @@ hipError_t launch_prop(int L, int path, int nr, const PropArgs<T> &A, const Solv
 }
 
 template <typename T>
-hipError_t launch_ktab(const GridDev<T> &g, int nchunk, int chunk_len, T thr2, const T *mut, T *ktab,
-                       hipStream_t st)
+hipError_t launch_ktab(const GridDev<T> &g, T thr2, const T *mut, T *ktab, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_ktab<T>, dim3(g.nblk, nchunk), dim3(kWave), 0, st, g, chunk_len, thr2, mut,
-                       ktab);
+    static int lds_max = -1;
+    if (lds_max < 0) {
+        int dev = 0, v = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+        if (e != hipSuccess) return e;
+        lds_max = v;
+    }
+    const int ntiles = g.nblk * (g.nblk + 1) / 2;
+    const size_t lds = (size_t)g.N * sizeof(T);
+    if (lds <= (size_t)lds_max) {
+        constexpr int threads = 1024;
+        if (lds > 65536) {
+            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_ktab<T, true>),
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+        }
+        const int blocks = (ntiles + threads / kWave - 1) / (threads / kWave);
+        hipLaunchKernelGGL((k_ktab<T, true>), dim3(blocks), dim3(threads), lds, st, g, ntiles, thr2, mut, ktab);
+    } else {
+        constexpr int threads = 256;
+        const int blocks = (ntiles + threads / kWave - 1) / (threads / kWave);
+        hipLaunchKernelGGL((k_ktab<T, false>), dim3(blocks), dim3(threads), 0, st, g, ntiles, thr2, mut, ktab);
+    }
     return hipGetLastError();
 }
 
@@ -621,10 +686,8 @@ template hipError_t launch_prop<double, double>(int, int, int, const PropArgs<do
                                                 const SolveDev<double, double> *, int, hipStream_t);
 template hipError_t launch_prop<float, float>(int, int, int, const PropArgs<float> &,
                                               const SolveDev<float, float> *, int, hipStream_t);
-template hipError_t launch_ktab<float>(const GridDev<float> &, int, int, float, const float *, float *,
-                                       hipStream_t);
-template hipError_t launch_ktab<double>(const GridDev<double> &, int, int, double, const double *,
-                                        double *, hipStream_t);
+template hipError_t launch_ktab<float>(const GridDev<float> &, float, const float *, float *, hipStream_t);
+template hipError_t launch_ktab<double>(const GridDev<double> &, double, const double *, double *, hipStream_t);
 template hipError_t launch_combine<float, double>(int, const CombArgs<float, double> &,
                                                   const SolveDev<float, double> *, int, hipStream_t);
 template hipError_t launch_combine<double, double>(int, const CombArgs<double, double> &,
