@@ -168,7 +168,19 @@ def main():
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--plain-sample", type=int, default=2)
+    ap.add_argument("--calib-copy", type=int, default=0,
+                    help="only run 3 plain device copies of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calib_copy:
+        import torch
+        a = torch.ones(args.calib_copy // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calib_copy_bytes": a.numel() * 4, "copies": 3}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (independent solves "
               "shard across GPUs with no communication; see DESIGN.md). Run with --gpus 1.",

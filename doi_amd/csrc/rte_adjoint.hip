@@ -348,6 +348,55 @@ __global__ void k_src_grad(int nx, int ny, int nz, double dxd, double dyd, doubl
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// The reference's forward-mode derivative (FisherInfo_IO/src/compute_grad_rte.cu:
+// 273-301 mu_a, 341-371 mu_s): the first term of solve i is its parameter's
+// region of the forward field phi = sum_q A W_q times -1 (mu_a) or g^l - 1
+// (mu_s), in scaled coefficients; its pixel buffers start at zero (no
+// ballistic term) with the bottom-layer stop-rule sums of that first term.
+// grid = (ceil(N/64), nsolve), block = 64.
+// ---------------------------------------------------------------------------
+template <typename T, typename TA, int L>
+__global__ __launch_bounds__(kWave) void k_ref_source(GridDev<T> g, Scales sc, const int *__restrict__ part,
+                                                     const int *__restrict__ param, int wrt, TA gan,
+                                                     const TA *__restrict__ phi, const TA *__restrict__ wsum,
+                                                     const SolveDev<T, TA> *__restrict__ sv)
+{
+    constexpr int K = (L + 1) * (L + 1);
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= g.N) return;
+    const SolveDev<T, TA> S = sv[blockIdx.y];
+    const bool in = part[v] == param[blockIdx.y];
+    TA gl = TA(1), bs = TA(0);
+#pragma unroll
+    for (int l = 0; l <= L; ++l) {
+        const TA f = wrt == 0 ? TA(-1) : gl - TA(1);
+#pragma unroll
+        for (int m = -l; m <= l; ++m) {
+            const int a = l * l + l + m;
+            const T wt = in ? T(f * phi[(size_t)v * K + a] * TA(sc.n[a])) : T(0);
+            S.W[(size_t)v * K + a] = wt;
+            bs += wsum[a] * (TA(wt) / TA(sc.n[a]));
+        }
+        gl *= gan;
+    }
+    const int P = g.nx * g.ny;
+    if (v >= (g.nz - 1) * P) {
+        const int pix = v - (g.nz - 1) * P;
+        S.E[pix] = TA(0);
+        S.dE[pix] = TA(0);
+        S.bsum[pix] = bs;
+    }
+}
+
+// y += x over n elements
+template <typename TA>
+__global__ void k_accumulate(size_t n, const TA *__restrict__ x, TA *__restrict__ y)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += x[i];
+}
+
 #define RTE_DISPATCH_L2(L, ...)                          \
     switch (L) {                                        \
     case 0: { constexpr int LL = 0; __VA_ARGS__; } break; \
@@ -422,6 +471,31 @@ hipError_t launch_src_grad(int L, int nx, int ny, int nz, double dx, double dy, 
     return hipGetLastError();
 }
 
+template <typename T, typename TA>
+hipError_t launch_ref_source(int L, const GridDev<T> &g, const Scales &sc, const int *part, const int *param, int wrt,
+                             TA gan, const TA *phi, const TA *wsum, const SolveDev<T, TA> *sv, int nsolve,
+                             hipStream_t st)
+{
+    dim3 grid((g.N + kWave - 1) / kWave, nsolve), block(kWave);
+    RTE_DISPATCH_L2(L, hipLaunchKernelGGL((k_ref_source<T, TA, LL>), grid, block, 0, st, g, sc, part, param, wrt,
+                                          gan, phi, wsum, sv));
+    return hipGetLastError();
+}
+
+template <typename TA>
+hipError_t launch_accumulate(size_t n, const TA *x, TA *y, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_accumulate<TA>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, x, y);
+    return hipGetLastError();
+}
+
+template hipError_t launch_ref_source<float, double>(int, const GridDev<float> &, const Scales &, const int *,
+                                                     const int *, int, double, const double *, const double *,
+                                                     const SolveDev<float, double> *, int, hipStream_t);
+template hipError_t launch_ref_source<double, double>(int, const GridDev<double> &, const Scales &, const int *,
+                                                      const int *, int, double, const double *, const double *,
+                                                      const SolveDev<double, double> *, int, hipStream_t);
+template hipError_t launch_accumulate<double>(size_t, const double *, double *, hipStream_t);
 template hipError_t launch_adj_prop<float, double>(int, const PropArgs<float> &, const float *, const float *,
                                                    float *, float *, const float *, hipStream_t);
 template hipError_t launch_adj_prop<double, double>(int, const PropArgs<double> &, const double *, const double *,

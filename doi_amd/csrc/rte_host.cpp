@@ -697,9 +697,20 @@ static void advance_series(HostSolve &h, double dT, double sumW, int rule, doubl
     if (stop || h.n >= max_terms) h.state = 2;
 }
 
+// First terms given on the device instead of beam sources: the reference's
+// derivative solves (one block parameter per solve, see launch_ref_source).
+template <typename TA>
+struct RefSrc {
+    const int *part = nullptr;   // device, N: parameter of each voxel (-1 none)
+    const int *param = nullptr;  // device, nb: parameter of each solve
+    int wrt = 0;                 // 0 mu_a, 1 mu_s
+    const TA *phi = nullptr;     // device, [N][K]: sum_q A W_q of the forward solve
+    const TA *wsum = nullptr;    // device, K: stop-rule weights
+};
+
 template <typename T, typename TA>
 static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_beam *beams,
-                     const rte_params *p, double *exitance, rte_stats *stats)
+                     const rte_params *p, double *exitance, rte_stats *stats, const RefSrc<TA> *rs = nullptr)
 {
     const rte_grid &grid = media[0]->grid;
     const int L = p->L, K = (L + 1) * (L + 1);
@@ -764,6 +775,8 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
         s.E = pix + 2 * P;
         s.red = ctx->ws_red.as<TA>() + 2 * i;
         s.mode = 1;
+        hs[i] = HostSolve{m, 0, 0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0};
+        if (rs) continue;
         // beam columns and power per column (CPU package/src/Dist.c:1090-1114, Inits.c:56)
         const double half = (beams[i].diam + 1.0e-16) / 2.0;
         int cols = 0;
@@ -778,7 +791,6 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
         if (cols == 0) return fail(RTE_E_ARG, "solve %d: beam hits no voxel column", i);
         bd[i] = BeamDev{beams[i].pos_x, beams[i].pos_y, half, beams[i].mag / (double)cols,
                         m->mua_d.as<double>(), m->mus_d.as<double>()};
-        hs[i] = HostSolve{m, 0, 0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0};
     }
     SolveDev<T, TA> *svd = ctx->ws_sv.as<SolveDev<T, TA>>();
     HIP_TRY(hipMemcpyAsync(svd, sv.data(), sizeof(SolveDev<T, TA>) * nb, hipMemcpyHostToDevice, st));
@@ -786,9 +798,18 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
     const Scales sc = make_scales(L);
     {
         LaunchTimer tm(ctx, 2);
-        HIP_TRY(launch_fill<T>(ctx->ws_W.as<T>(), szW * nb, T(0), st));
-        HIP_TRY(launch_source<T, TA>(L, g, dx, dy, dz, grid.xmin, grid.ymin, sc, ctx->ws_beams.as<BeamDev>(),
-                                     svd, nb, st));
+        if (rs) {
+            if constexpr (sizeof(TA) == 8) {
+                HIP_TRY(launch_ref_source<T, TA>(L, g, sc, rs->part, rs->param, rs->wrt, TA(T(media[0]->g)), rs->phi,
+                                                 rs->wsum, svd, nb, st));
+            } else {
+                return fail(RTE_E_UNSUPP, "derivative solves need FP64 running sums");
+            }
+        } else {
+            HIP_TRY(launch_fill<T>(ctx->ws_W.as<T>(), szW * nb, T(0), st));
+            HIP_TRY(launch_source<T, TA>(L, g, dx, dy, dz, grid.xmin, grid.ymin, sc, ctx->ws_beams.as<BeamDev>(),
+                                         svd, nb, st));
+        }
         HIP_TRY(launch_bottom<T, TA>(P, svd, nb, st));
         tm.done();
     }
@@ -1359,35 +1380,93 @@ extern "C" int rte_gradient(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, c
 static int gram_mfma(rte_ctx *ctx, int P, int n, const float *J, const float *w, float *G);
 
 // ---- Fisher information ---------------------------------------------------
+// The reference's forward-mode Jacobian of the exitance with respect to block
+// parameters (FisherInfo_IO/src/compute_grad_rte.cu:273-301 mu_a, 341-371 mu_s;
+// GPU package/src/inverse_rte.cu:290-433): one forward solve keeping
+// phi = sum_q A W_q, then one Neumann series per parameter from phi restricted
+// to the parameter's voxels (times -1 for mu_a, g^l - 1 for mu_s), all
+// parameters as one batch on the shared medium; column a of J is that series'
+// exitance without the ballistic term.  Not the exact derivative of the
+// discrete model: the reference approximates dA/dmu by -A chi A.
+template <typename T, typename TA>
+static int ref_jacobian_run(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte_params *p, const int *part,
+                            int nparam, int wrt, std::vector<double> &J, std::vector<double> &E, rte_stats *stats)
+{
+    AdjointRun<T, TA> run;
+    int rc;
+    if ((rc = run.forward(ctx, m, beam, p, 1))) return rc;
+    const int P = run.P, N = run.N, K = run.K;
+    hipStream_t st = ctx->stream;
+    DevBuf dPhi, dPart, dParam, dWs;
+    HIP_TRY(dPhi.alloc(run.F * sizeof(TA)));
+    HIP_TRY(dPart.alloc(sizeof(int) * N));
+    HIP_TRY(dParam.alloc(sizeof(int) * nparam));
+    HIP_TRY(dWs.alloc(sizeof(TA) * K));
+    HIP_TRY(launch_fill<TA>(dPhi.as<TA>(), run.F, TA(0), st));
+    for (int q = 0; q < run.n; ++q) HIP_TRY(launch_accumulate<TA>(run.F, run.ckpt[q]->template as<TA>(), dPhi.template as<TA>(), st));
+    std::vector<int> prm(nparam);
+    for (int a = 0; a < nparam; ++a) prm[a] = a;
+    std::vector<TA> ws(K);
+    for (int a = 0; a < K; ++a) ws[a] = TA(stop_weight(a));
+    HIP_TRY(hipMemcpyAsync(dPart.p, part, sizeof(int) * N, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(dParam.p, prm.data(), sizeof(int) * nparam, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(dWs.p, ws.data(), sizeof(TA) * K, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    RefSrc<TA> rs;
+    rs.part = dPart.as<int>();
+    rs.param = dParam.as<int>();
+    rs.wrt = wrt;
+    rs.phi = dPhi.as<TA>();
+    rs.wsum = dWs.as<TA>();
+    std::vector<rte_medium *> med(nparam, m);
+    std::vector<double> img((size_t)nparam * P);
+    if ((rc = run_batch<T, TA>(ctx, nparam, med.data(), nullptr, p, img.data(), nullptr, &rs))) return rc;
+    J.assign((size_t)P * nparam, 0.0);
+    for (int a = 0; a < nparam; ++a)
+        for (int pix = 0; pix < P; ++pix) J[(size_t)pix * nparam + a] = img[(size_t)a * P + pix];
+    E.assign(run.E.begin(), run.E.end());
+    run.fill_stats(stats);
+    return RTE_OK;
+}
+
 // J[p][a] = d E_p / d theta_a with theta_a = mu_a (or mu_s) on the voxels of
-// partition a, one backward pass per detector; F = J^T diag(1/E) J on FP32
+// partition a: exact (method 0, one backward pass per detector) or the
+// reference's forward-mode derivative (method 1); F = J^T diag(1/E) J on FP32
 // MFMA (detectors with E = 0 skipped, FisherInfo_IO/src/rte_fim.cu:318-324);
 // lower Cholesky factor of F in FP64.
 template <typename T, typename TA>
 static int fisher_run(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte_params *p, int stride,
-                      const int *part, int nparam, int wrt, double *Fout, double *chol, double *Jout, double *exitance,
-                      rte_stats *stats)
+                      int method, const int *part, int nparam, int wrt, double *Fout, double *chol, double *Jout,
+                      double *exitance, rte_stats *stats)
 {
-    AdjointRun<T, TA> run;
     int rc;
-    if ((rc = run.forward(ctx, m, beam, p, stride))) return rc;
-    const int P = run.P, N = run.N;
-    std::vector<double> J((size_t)P * nparam, 0.0), cot(P, 0.0), ga(N), gs(N);
-    for (int pix = 0; pix < P; ++pix) {
-        if (exitance) exitance[pix] = (double)run.E[pix];
-        if (!((double)run.E[pix] != 0.0)) continue;
-        cot.assign(P, 0.0);
-        cot[pix] = 1.0;
-        if ((rc = run.backward(cot.data(), ga.data(), gs.data()))) return rc;
-        const std::vector<double> &gv = wrt == 0 ? ga : gs;
-        for (int v = 0; v < N; ++v)
-            if (part[v] >= 0) J[(size_t)pix * nparam + part[v]] += gv[v];
+    const int P = m->grid.nx * m->grid.ny, N = m->N;
+    std::vector<double> J((size_t)P * nparam, 0.0), E(P, 0.0);
+    if (method == 1) {
+        if ((rc = ref_jacobian_run<T, TA>(ctx, m, beam, p, part, nparam, wrt, J, E, stats))) return rc;
+    } else {
+        // exact reverse mode: one adjoint series per detector
+        AdjointRun<T, TA> run;
+        if ((rc = run.forward(ctx, m, beam, p, stride))) return rc;
+        std::vector<double> cot(P, 0.0), ga(N), gs(N);
+        for (int pix = 0; pix < P; ++pix) {
+            E[pix] = (double)run.E[pix];
+            if (!(E[pix] != 0.0)) continue;
+            cot.assign(P, 0.0);
+            cot[pix] = 1.0;
+            if ((rc = run.backward(cot.data(), ga.data(), gs.data()))) return rc;
+            const std::vector<double> &gv = wrt == 0 ? ga : gs;
+            for (int v = 0; v < N; ++v)
+                if (part[v] >= 0) J[(size_t)pix * nparam + part[v]] += gv[v];
+        }
+        run.fill_stats(stats);
     }
+    if (exitance) std::memcpy(exitance, E.data(), sizeof(double) * P);
     if (Jout) std::memcpy(Jout, J.data(), sizeof(double) * J.size());
     // Gram on MFMA (FP32 inputs, FP32 accumulation)
     std::vector<float> Jf((size_t)P * nparam), wf(P);
     for (size_t i = 0; i < Jf.size(); ++i) Jf[i] = (float)J[i];
-    for (int pix = 0; pix < P; ++pix) wf[pix] = (double)run.E[pix] != 0.0 ? (float)(1.0 / (double)run.E[pix]) : 0.f;
+    for (int pix = 0; pix < P; ++pix) wf[pix] = E[pix] != 0.0 ? (float)(1.0 / E[pix]) : 0.f;
     std::vector<float> Ff((size_t)nparam * nparam);
     if ((rc = gram_mfma(ctx, P, nparam, Jf.data(), wf.data(), Ff.data()))) return rc;
     // the two triangles come from different MFMA tiles: symmetrise
@@ -1410,7 +1489,6 @@ static int fisher_run(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const r
             }
         std::memcpy(chol, C.data(), sizeof(double) * C.size());
     }
-    run.fill_stats(stats);
     return RTE_OK;
 }
 
@@ -1424,11 +1502,35 @@ extern "C" int rte_fisher(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, con
     if ((rc = grad_check(ctx, m, beam, params, opts, &stride))) return rc;
     for (int v = 0; v < m->N; ++v)
         if (partition[v] >= nparam) return fail(RTE_E_ARG, "rte_fisher: partition[%d] = %d >= nparam", v, partition[v]);
+    const int method = opts ? opts->jacobian : 0;
+    if (method != 0 && method != 1) return fail(RTE_E_ARG, "rte_fisher: jacobian must be 0 (exact) or 1 (reference)");
     if (params->precision == RTE_PREC_FP64)
-        return fisher_run<double, double>(ctx, m, beam, params, stride, partition, nparam, wrt, F, chol, J, exitance,
-                                          stats);
-    return fisher_run<float, double>(ctx, m, beam, params, stride, partition, nparam, wrt, F, chol, J, exitance,
-                                     stats);
+        return fisher_run<double, double>(ctx, m, beam, params, stride, method, partition, nparam, wrt, F, chol, J,
+                                          exitance, stats);
+    return fisher_run<float, double>(ctx, m, beam, params, stride, method, partition, nparam, wrt, F, chol, J,
+                                     exitance, stats);
+}
+
+extern "C" int rte_ref_jacobian(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte_params *params,
+                                const int *partition, int nparam, int wrt, double *J, double *exitance,
+                                rte_stats *stats)
+{
+    if (!partition || !J || nparam < 1) return fail(RTE_E_ARG, "rte_ref_jacobian: bad partition/nparam/J");
+    if (wrt != 0 && wrt != 1) return fail(RTE_E_ARG, "rte_ref_jacobian: wrt must be 0 (mu_a) or 1 (mu_s)");
+    int stride, rc;
+    if ((rc = grad_check(ctx, m, beam, params, nullptr, &stride))) return rc;
+    for (int v = 0; v < m->N; ++v)
+        if (partition[v] >= nparam)
+            return fail(RTE_E_ARG, "rte_ref_jacobian: partition[%d] = %d >= nparam", v, partition[v]);
+    std::vector<double> Jv, E;
+    if (params->precision == RTE_PREC_FP64)
+        rc = ref_jacobian_run<double, double>(ctx, m, beam, params, partition, nparam, wrt, Jv, E, stats);
+    else
+        rc = ref_jacobian_run<float, double>(ctx, m, beam, params, partition, nparam, wrt, Jv, E, stats);
+    if (rc) return rc;
+    std::memcpy(J, Jv.data(), sizeof(double) * Jv.size());
+    if (exitance) std::memcpy(exitance, E.data(), sizeof(double) * E.size());
+    return RTE_OK;
 }
 
 // Gram G = J^T diag(w) J of a P x n matrix on MFMA (rte_kernels.hip: k_gram)

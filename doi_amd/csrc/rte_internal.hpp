@@ -154,6 +154,12 @@ template <typename T, typename TA>
 hipError_t launch_adj_init(int L, const GridDev<T> &g, const TA *cot, const Scales &sc, const TA *coefE, T *X,
                            hipStream_t st);
 template <typename T, typename TA>
+hipError_t launch_ref_source(int L, const GridDev<T> &g, const Scales &sc, const int *part, const int *param, int wrt,
+                             TA gan, const TA *phi, const TA *wsum, const SolveDev<T, TA> *sv, int nsolve,
+                             hipStream_t st);
+template <typename TA>
+hipError_t launch_accumulate(size_t n, const TA *x, TA *y, hipStream_t st);
+template <typename T, typename TA>
 hipError_t launch_make_wq(int L, const GridDev<T> &g, const Scales &sc, TA gan, const T *mus, const TA *aw_prev,
                           T *Wq, hipStream_t st);
 template <typename T, typename TA>

@@ -58,7 +58,7 @@ class Stats(C.Structure):
 
 
 class GradOpts(C.Structure):
-    _fields_ = [("checkpoint_stride", C.c_int)]
+    _fields_ = [("checkpoint_stride", C.c_int), ("jacobian", C.c_int)]
 
 
 class Prof(C.Structure):
@@ -108,6 +108,8 @@ SIGNATURES = {
                                _dp, _dp, C.POINTER(Stats)]),
     "rte_fisher": (C.c_int, [_vp, _vp, C.POINTER(Beam), C.POINTER(Params), C.POINTER(GradOpts), _ip, C.c_int,
                              C.c_int, _dp, _dp, _dp, _dp, C.POINTER(Stats)]),
+    "rte_ref_jacobian": (C.c_int, [_vp, _vp, C.POINTER(Beam), C.POINTER(Params), _ip, C.c_int, C.c_int, _dp, _dp,
+                                   C.POINTER(Stats)]),
     "rte_server_create": (C.c_int, [_vp, C.POINTER(ServerOpts), C.POINTER(_vp)]),
     "rte_server_destroy": (C.c_int, [_vp]),
     "rte_submit": (C.c_int, [_vp, C.POINTER(Grid), C.POINTER(Optics), C.POINTER(Beam), C.POINTER(Params),
@@ -260,7 +262,7 @@ class Context:
         gs = np.zeros(medium.N)
         st = Stats()
         _check(lib().rte_exitance_vjp(self.h, medium.h, C.byref(Beam(*beam)), C.byref(params),
-                                      C.byref(GradOpts(checkpoint_stride)), _p(cot), _p(img), _p(ga), _p(gs),
+                                      C.byref(GradOpts(checkpoint_stride, 0)), _p(cot), _p(img), _p(ga), _p(gs),
                                       C.byref(st)))
         return img.reshape(ny, nx), ga, gs, dict(terms=st.terms, gpu_ms=st.gpu_ms)
 
@@ -273,11 +275,12 @@ class Context:
         loss = C.c_double(0.0)
         st = Stats()
         _check(lib().rte_gradient(self.h, medium.h, C.byref(Beam(*beam)), C.byref(params),
-                                  C.byref(GradOpts(checkpoint_stride)), _p(tgt), C.byref(loss), _p(img), _p(ga),
+                                  C.byref(GradOpts(checkpoint_stride, 0)), _p(tgt), C.byref(loss), _p(img), _p(ga),
                                   _p(gs), C.byref(st)))
         return loss.value, img.reshape(ny, nx), ga, gs, dict(terms=st.terms, gpu_ms=st.gpu_ms)
 
-    def fisher(self, medium, beam, params, partition, nparam, wrt=0, checkpoint_stride=1):
+    def fisher(self, medium, beam, params, partition, nparam, wrt=0, checkpoint_stride=1, jacobian=0):
+        """jacobian 0: exact (adjoint per detector); 1: the reference's forward-mode derivative."""
         nx, ny, _ = medium.shape
         part = np.ascontiguousarray(np.ravel(partition), dtype=np.int32)
         F = np.zeros((nparam, nparam))
@@ -286,9 +289,20 @@ class Context:
         img = np.zeros(nx * ny)
         st = Stats()
         _check(lib().rte_fisher(self.h, medium.h, C.byref(Beam(*beam)), C.byref(params),
-                                C.byref(GradOpts(checkpoint_stride)), part.ctypes.data_as(_ip), nparam, wrt,
-                                _p(F), _p(Cf), _p(J), _p(img), C.byref(st)))
+                                C.byref(GradOpts(checkpoint_stride, jacobian)), part.ctypes.data_as(_ip), nparam,
+                                wrt, _p(F), _p(Cf), _p(J), _p(img), C.byref(st)))
         return F, Cf, J, img.reshape(ny, nx), dict(terms=st.terms, gpu_ms=st.gpu_ms)
+
+    def ref_jacobian(self, medium, beam, params, partition, nparam, wrt=0):
+        """The reference's forward-mode derivative images, J[p, a] (rte_ref_jacobian)."""
+        nx, ny, _ = medium.shape
+        part = np.ascontiguousarray(np.ravel(partition), dtype=np.int32)
+        J = np.zeros((nx * ny, nparam))
+        img = np.zeros(nx * ny)
+        st = Stats()
+        _check(lib().rte_ref_jacobian(self.h, medium.h, C.byref(Beam(*beam)), C.byref(params),
+                                      part.ctypes.data_as(_ip), nparam, wrt, _p(J), _p(img), C.byref(st)))
+        return J, img.reshape(ny, nx), dict(terms=st.terms, gpu_ms=st.gpu_ms)
 
     # ---- operators -------------------------------------------------------
     def propagate(self, shape, bounds, mu_a, mu_s, g, W, L, prop_thresh=0.0, precision=PREC_FP64,

@@ -175,6 +175,9 @@ int rte_solve_media(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_be
 typedef struct {
     int checkpoint_stride; /* keep A W_q of every stride-th term of the forward pass and
                               recompute the others in the backward pass (>= 1; 1 keeps all) */
+    int jacobian;          /* rte_fisher: 0 the exact derivative (one adjoint series per
+                              detector), 1 the reference's forward-mode derivative
+                              (rte_ref_jacobian: one series per parameter) */
 } rte_grad_opts;
 
 /* dmu_a[v] = sum_p cotangent[p] dE_p/dmu_a(v), same for dmu_s (N each, may be NULL);
@@ -197,6 +200,19 @@ int rte_gradient(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte_pa
 int rte_fisher(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte_params *params,
                const rte_grad_opts *opts, const int *partition, int nparam, int wrt, double *F,
                double *chol, double *J, double *exitance, rte_stats *stats);
+
+/* The reference's own derivative of the exitance with respect to block parameters
+ * (FisherInfo_IO/src/compute_grad_rte.cu:273-301 mu_a, 341-371 mu_s; GPU package/src/
+ * inverse_rte.cu:290-433): phi = A sum_q W_q of the forward solve; per parameter a, one
+ * Neumann series from c_m (-1) phi (wrt 0) or c_m (g^l - 1) phi (wrt 1) on the voxels with
+ * partition[v] == a, and J[p][a] = its exitance without the ballistic term (nx*ny x nparam).
+ * The reference approximates dA/dmu by -A chi A and ignores the ballistic and source terms,
+ * so this is not the exact derivative rte_exitance_vjp gives; it reproduces the reference's
+ * gradients (the mu_s region applied as intended: the reference indexes it with a stale
+ * voxel index, compute_grad_rte.cu:357-366).  All parameters run as one batch. */
+int rte_ref_jacobian(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte_params *params,
+                     const int *partition, int nparam, int wrt, double *J, double *exitance,
+                     rte_stats *stats);
 
 /* ---- serving --------------------------------------------------------------
  * Continuous batching (no counterpart in the reference, which runs one solve

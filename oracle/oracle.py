@@ -72,6 +72,8 @@ def lib():
                                 C.POINTER(Params), dp, C.POINTER(Stats)]
         L.orc_solve_field.argtypes = [C.POINTER(Grid), C.POINTER(Optics), C.POINTER(Beam),
                                       C.POINTER(Params), dp, dp, C.POINTER(Stats)]
+        L.orc_ref_derivative.argtypes = [C.POINTER(Grid), C.POINTER(Optics), C.POINTER(Beam), C.POINTER(Params),
+                                         C.POINTER(C.c_int), C.c_int, dp, C.POINTER(Stats)]
         L.orc_bottom_sum.argtypes = [C.POINTER(Grid), C.c_int, dp]
         L.orc_bottom_sum.restype = C.c_double
         _LIB = L
@@ -121,6 +123,22 @@ class Problem:
         if field:
             out["w"] = w.reshape((L + 1) ** 2, self.N)
         return out
+
+    def ref_derivative(self, beam, L, region, which, stop_rule=STOP_LAMBDA, tol=1e-6, max_terms=1000,
+                       prop_thresh=1e30, exit_quad=QUAD_REF):
+        """The reference's forward-mode derivative image of the exitance with
+        respect to mu_a (which=0) or mu_s (which=1) on the voxels where region != 0."""
+        nx, ny, _ = self.shape
+        reg = np.ascontiguousarray(np.ravel(region), dtype=np.int32)
+        assert reg.size == self.N
+        img = np.zeros(nx * ny)
+        st = Stats()
+        prm = Params(L, max_terms, stop_rule, tol, prop_thresh, exit_quad)
+        rc = lib().orc_ref_derivative(C.byref(self.grid), C.byref(self.optics), C.byref(Beam(*beam)), C.byref(prm),
+                                      reg.ctypes.data_as(C.POINTER(C.c_int)), which, _p(img), C.byref(st))
+        if rc != 0:
+            raise RuntimeError("orc_ref_derivative failed: %d" % rc)
+        return img.reshape(ny, nx), st.terms
 
     def source(self, beam, L):
         K = (L + 1) ** 2

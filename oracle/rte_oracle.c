@@ -615,19 +615,20 @@ static int stop_lambda(double *fi, int n, double sumW, double sumOut, double *la
     return hi > 0.9999;
 }
 
-static int solve_impl(const orc_grid *g, const orc_optics *o, const orc_beam *b, const orc_params *p,
-                      double *exitance, double *w_out, orc_stats *st)
+/* The Neumann series from the first term s0 (K*N): exitance = ball + the
+ * bottom exitance of A sum_q (DA)^q s0 (ball may be NULL: no ballistic term);
+ * w_out (may be NULL) receives A sum_q (DA)^q s0. */
+static int neumann_from(const orc_grid *g, const orc_optics *o, const orc_params *p, const double *s0,
+                        const double *ball, double *exitance, double *w_out, orc_stats *st)
 {
     int L = p->L, K = ncoef(L), N = g->nx * g->ny * g->nz, P = g->nx * g->ny;
-    double *s0 = malloc(sizeof(double) * (size_t)K * N);
-    double *ball = malloc(sizeof(double) * P);
-    if (!orc_source(g, o, b, L, s0, ball)) { free(s0); free(ball); return -1; }
     double *W = malloc(sizeof(double) * (size_t)K * N);
     double *AW = malloc(sizeof(double) * (size_t)K * N);
     double *out = malloc(sizeof(double) * (size_t)K * N);
     double *img = malloc(sizeof(double) * P);
     double btot = 0.0;
-    for (int i = 0; i < P; ++i) btot += ball[i];
+    if (ball)
+        for (int i = 0; i < P; ++i) btot += ball[i];
     memcpy(W, s0, sizeof(double) * (size_t)K * N);
     memcpy(out, s0, sizeof(double) * (size_t)K * N);
     /* with g == 0 only l = 0 survives the scatter, and the reference propagates
@@ -650,13 +651,13 @@ static int solve_impl(const orc_grid *g, const orc_optics *o, const orc_beam *b,
         }
         orc_propagate(g, o, L, p->prop_thresh, out, AW);
         orc_exitance(g, o, L, p->exit_quad, AW, img);
-        for (int i = 0; i < P; ++i) exitance[i] = ball[i] + img[i];
+        for (int i = 0; i < P; ++i) exitance[i] = (ball ? ball[i] : 0.0) + img[i];
         if (w_out) memcpy(w_out, AW, sizeof(double) * (size_t)K * N);
     } else {
         /* relative change of the total exitance: by linearity of A the exitance
          * of A out_n is the running sum of the exitances of A W_q */
         double T = btot;
-        for (int i = 0; i < P; ++i) exitance[i] = ball[i];
+        for (int i = 0; i < P; ++i) exitance[i] = ball ? ball[i] : 0.0;
         if (w_out) memset(w_out, 0, sizeof(double) * (size_t)K * N);
         n = 0;
         for (;;) {
@@ -679,14 +680,58 @@ static int solve_impl(const orc_grid *g, const orc_optics *o, const orc_beam *b,
         st->rel_change_last = rel;
         st->ballistic_total = btot;
     }
-    free(s0); free(ball); free(W); free(AW); free(out); free(img);
+    free(W); free(AW); free(out); free(img);
     return 0;
+}
+
+static int solve_impl(const orc_grid *g, const orc_optics *o, const orc_beam *b, const orc_params *p,
+                      double *exitance, double *w_out, orc_stats *st)
+{
+    int L = p->L, K = ncoef(L), N = g->nx * g->ny * g->nz, P = g->nx * g->ny;
+    double *s0 = malloc(sizeof(double) * (size_t)K * N);
+    double *ball = malloc(sizeof(double) * P);
+    int rc = -1;
+    if (orc_source(g, o, b, L, s0, ball)) rc = neumann_from(g, o, p, s0, ball, exitance, w_out, st);
+    free(s0);
+    free(ball);
+    return rc;
 }
 
 int orc_solve(const orc_grid *g, const orc_optics *o, const orc_beam *b, const orc_params *p,
               double *exitance, orc_stats *st)
 {
     return solve_impl(g, o, b, p, exitance, NULL, st);
+}
+
+/* FisherInfo_IO/src/compute_grad_rte.cu:241-301 (mu_a) and :341-371 (mu_s):
+ * out2 = the forward field A sum_q (DA)^q s0; the derivative source is out2
+ * restricted to the region and scaled by -C (mu_a) or by -C + g^l (mu_s,
+ * PropScatmu1, GPU package/src/dist.cu:675-686); Neumann(src) and the
+ * transmitted image without the ballistic term (generate_trans_image(out3, 0)).
+ * C is the speed in the medium of the reference's units (C = 1 with n = 1 in
+ * FisherInfo_IO/src/rte.h:15, whose scatter carries no speed factor); with this
+ * restatement's c_m in A (1/c_m) and D (c_m) the same derivative has the source
+ * c_m (-1) phi|_R and c_m (g^l - 1) phi|_R.  The region is applied as
+ * intended in both branches (the reference's mu_s branch indexes the region
+ * with a stale voxel index, compute_grad_rte.cu:357-366). */
+int orc_ref_derivative(const orc_grid *g, const orc_optics *o, const orc_beam *b, const orc_params *p,
+                       const int *region, int which, double *dimg, orc_stats *st)
+{
+    int L = p->L, K = ncoef(L), N = g->nx * g->ny * g->nz, P = g->nx * g->ny;
+    double *phi = malloc(sizeof(double) * (size_t)K * N);
+    double *img = malloc(sizeof(double) * P);
+    int rc = solve_impl(g, o, b, p, img, phi, NULL);
+    if (rc == 0) {
+        for (int a = 0; a < K; ++a) {
+            const int l = (int)sqrt((double)a + 0.5);
+            const double f = (o->c / o->n) * (which == 0 ? -1.0 : pow(o->g, (double)l) - 1.0);
+            for (int v = 0; v < N; ++v) phi[(size_t)a * N + v] = region[v] ? f * phi[(size_t)a * N + v] : 0.0;
+        }
+        rc = neumann_from(g, o, p, phi, NULL, dimg, NULL, st);
+    }
+    free(phi);
+    free(img);
+    return rc;
 }
 
 int orc_solve_field(const orc_grid *g, const orc_optics *o, const orc_beam *b, const orc_params *p,

@@ -102,6 +102,13 @@ int orc_solve_field(const orc_grid *g, const orc_optics *o, const orc_beam *b,
                     const orc_params *p, double *exitance, double *w_out,
                     orc_stats *st);
 
+/* the reference's forward-mode derivative of the exitance with respect to
+ * mu_a (which = 0) or mu_s (which = 1) of the voxels with region[v] != 0:
+ * one Neumann series from the forward field restricted to the region, its
+ * exitance without the ballistic term (nx*ny) */
+int orc_ref_derivative(const orc_grid *g, const orc_optics *o, const orc_beam *b, const orc_params *p,
+                       const int *region, int which, double *dimg, orc_stats *st);
+
 /* bottom-layer sums of the stop rule (complex-basis real parts) */
 double orc_bottom_sum(const orc_grid *g, int L, const double *W);
 

@@ -145,3 +145,44 @@ def test_fisher_information(ctx):
     assert np.array_equal(F, F.T)
     assert np.allclose(np.tril(Cf), Cf)
     assert np.max(np.abs(Cf @ Cf.T - F)) < 1e-12 * np.max(np.abs(F))
+
+
+def test_reference_jacobian_matches_oracle(ctx):
+    """rte_ref_jacobian (the reference's own forward-mode derivative) against the
+    oracle's restatement of it, with every series held at the same term count:
+    FP64 within 1e-9 and mixed precision within 1e-4 of max |J|; Fisher built on
+    it.  The gap to the exact derivative is reported, not bounded: the reference
+    approximates dA/dmu by -A chi A."""
+    shape = (6, 6, 5)
+    bounds = bounds_for(shape)
+    ma, ms = hetero_maps(shape, seed=23, sigma=0.3)
+    i, j, k = np.meshgrid(np.arange(6), np.arange(6), np.arange(5), indexing="ij")
+    part = ((i // 3) + 2 * (j // 3) + 4 * (k // 3)).transpose(2, 1, 0).ravel()  # 8 blocks
+    nparam, n = 8, 12
+    med = ctx.medium(shape, bounds, ma, ms, G)
+    P = O.Problem(shape, bounds, ma, ms, G)
+    for wrt in (0, 1):
+        p64 = rte.make_params(L, precision=rte.PREC_FP64, stop_rule=rte.STOP_DELTA, tol=0.0, max_terms=n)
+        J, img, st = ctx.ref_jacobian(med, BEAM, p64, part, nparam, wrt)
+        assert st["terms"] == n
+        sc = np.max(np.abs(J))
+        for a in range(nparam):
+            ref, terms = P.ref_derivative(BEAM, L, (part == a).astype(np.int32), wrt, stop_rule=O.STOP_DELTA,
+                                          tol=0.0, max_terms=n)
+            assert terms == n
+            assert np.max(np.abs(J[:, a] - ref.ravel())) < 1e-9 * sc, (wrt, a)
+        pm = rte.make_params(L, precision=rte.PREC_MIXED, stop_rule=rte.STOP_DELTA, tol=0.0, max_terms=n)
+        Jm, _, _ = ctx.ref_jacobian(med, BEAM, pm, part, nparam, wrt)
+        assert np.max(np.abs(Jm - J)) < 1e-4 * sc
+        # exact derivative of the same blocks, for the record
+        _, _, Jx, _, _ = ctx.fisher(med, BEAM, p64, part, nparam, wrt=wrt, jacobian=0)
+        gap = np.linalg.norm(J - Jx) / np.linalg.norm(Jx)
+        print("wrt %d: |J_ref - J_exact| / |J_exact| = %.3f" % (wrt, gap))
+        assert np.isfinite(gap)
+    F, Cf, Jf, img, _ = ctx.fisher(med, BEAM, p64, part, nparam, wrt=0, jacobian=1)
+    J0, _, _ = ctx.ref_jacobian(med, BEAM, p64, part, nparam, 0)
+    assert np.max(np.abs(Jf - J0)) <= 1e-12 * np.max(np.abs(J0))
+    E = img.ravel()
+    F64 = Jf.T @ (Jf / E[:, None])
+    assert np.max(np.abs(F - F64)) < 2e-5 * np.max(np.abs(F64))
+    med.close()

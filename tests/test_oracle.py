@@ -302,3 +302,47 @@ def test_config2_fixture_reproduces():
     lam = [x for x in fx["cases"] if x["name"] == "lambda_ref"][0]
     assert lam["terms"] == 10
     assert len(lam["exitance"]) == 16 and len(lam["exitance"][0]) == 16
+
+
+def _ref_derivative_by_operators(P, beam, L, region, which, n):
+    """The reference's forward-mode derivative composed from the oracle's
+    separately pinned operators: phi = A out of the forward solve, the
+    derivative source c_m f_l phi on the region (f_l = -1 for mu_a, g^l - 1
+    for mu_s), n Neumann terms, exitance without the ballistic term."""
+    K = (L + 1) ** 2
+    fwd = P.solve(beam, L, stop_rule=O.STOP_DELTA, tol=0.0, max_terms=n, field=True)
+    phi = fwd["w"]
+    c_m = P.optics.c / P.optics.n
+    ls = np.floor(np.sqrt(np.arange(K) + 0.5)).astype(int)
+    f = -np.ones(K) if which == 0 else P.optics.g ** ls - 1.0
+    W = c_m * f[:, None] * phi * (np.ravel(region) != 0)[None, :]
+    E = np.zeros(P.shape[0] * P.shape[1])
+    for _ in range(n):
+        AW = P.propagate(W, L)
+        E += P.exitance(AW, L)
+        W = P.scatter(AW, L)
+    return E.reshape(P.shape[1], P.shape[0])
+
+
+@pytest.mark.parametrize("which", [0, 1])
+def test_reference_derivative_composition_and_additivity(which):
+    shape, L, g = (5, 4, 4), 2, 0.7
+    bounds = bounds_for(shape)
+    ma, ms = hetero_maps(shape, seed=21, sigma=0.3)
+    P = O.Problem(shape, bounds, ma, ms, g, n=1.3)
+    beam = (0.0, 0.05, 0.1, 1.0)
+    N = int(np.prod(shape))
+    rng = np.random.default_rng(5)
+    lab = rng.integers(0, 3, N)
+    n = 6
+    d = {}
+    for r in range(3):
+        d[r], terms = P.ref_derivative(beam, L, (lab == r).astype(np.int32), which, stop_rule=O.STOP_DELTA,
+                                       tol=0.0, max_terms=n)
+        assert terms == n
+        ref = _ref_derivative_by_operators(P, beam, L, lab == r, which, n)
+        assert rel_max(d[r], ref) < 1e-12
+    # the derivative source is linear in the region
+    u, _ = P.ref_derivative(beam, L, (lab != 1).astype(np.int32), which, stop_rule=O.STOP_DELTA, tol=0.0,
+                            max_terms=n)
+    assert rel_max(u, d[0] + d[2]) < 1e-12
