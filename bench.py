@@ -9,6 +9,7 @@ in HBM before the timed region; every step rebuilds their path tables.
 
     python bench.py [--gpus 1] [--steps 2] [--warmup 1]
     python bench.py --serve [--requests 2000] [--rate 200]   # serving stream (config 4)
+    python bench.py --config3                                # gradient + Fisher at 24^3
 
 Prints one JSON line.  One GPU only: --gpus N > 1 exits with a message.
 """
@@ -25,6 +26,10 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 (vector/matrix dense) published peak
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per solve and term of k_prop at 32^3 L=5 (mixed, cached), from rocprofv3
+# FETCH_SIZE + WRITE_SIZE calibrated against a device copy
+# (profiles/k_prop_32cube_L5_4media_hbm_traffic.md): a beam in a group of two, a single beam
+TRAFFIC_PER_SOLVE_TERM = {2: 2.41e9, 1: 4.30e9}
 
 
 def log(*a):
@@ -152,6 +157,51 @@ def serve(args):
     ctx.close()
 
 
+def config3(args):
+    """Differentiation workload: 24^3 heterogeneous medium, L=5, mixed
+    precision; the loss gradient with respect to every voxel's mu_a and mu_s
+    (one forward + one adjoint series) and the Fisher matrix of 64 block
+    parameters on 576 detectors (J by the reference's per-parameter derivative
+    series, Gram on MFMA, FP64 Cholesky)."""
+    from doi_amd import media, rte
+    n, L, g = 24, 5, 0.8
+    shape, bounds = (n, n, n), media.cube_bounds(n)
+    maps = lambda seed: media.heterogeneous(n, seed, sigma=0.2, n_incl=1, radius=(3, 3), factor=(2.0, 2.0))
+    ma, ms = maps(3)
+    ma2, ms2 = maps(4)
+    beam = (0.05, 0.05, 0.2, 1.0)
+    ctx = rte.Context(0)
+    pm = rte.make_params(L, precision=rte.PREC_MIXED, stop_rule=rte.STOP_DELTA, tol=1e-6, max_terms=30)
+    target, _ = ctx.solve(shape, bounds, ma2, ms2, g, beam, pm)
+    med = ctx.medium(shape, bounds, ma, ms, g)
+    i, j, k = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    part = ((i // 6) + 4 * (j // 6) + 16 * (k // 6)).transpose(2, 1, 0).ravel()
+    ctx.gradient(med, beam, pm, target)          # warm-up (tables, moments)
+    tg, tf = [], []
+    for _ in range(max(args.steps, 1)):
+        med.reset_cache()
+        t0 = time.perf_counter()
+        loss, img, ga, gs, st = ctx.gradient(med, beam, pm, target)
+        tg.append(time.perf_counter() - t0)
+        med.reset_cache()
+        t0 = time.perf_counter()
+        F, Cf, J, _, stf = ctx.fisher(med, beam, pm, part, 64, wrt=0, jacobian=1)
+        tf.append(time.perf_counter() - t0)
+        log("gradient %.3f s, Fisher %.3f s" % (tg[-1], tf[-1]))
+    ev = np.linalg.eigvalsh(F)
+    res = {"metric": "config3: gradient and Fisher time at 24^3 L=5", "value": 1.0 / float(np.mean(tg)),
+           "unit": "gradients/s", "n_gpus": 1, "steps": len(tg), "warmup": 1,
+           "ms_per_step": 1000.0 * float(np.mean(tg)), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "fp32 (fp64 running sums)", "data": "synthetic (seeded, config 3)",
+           "config": {"workload": "config3: 24^3, L=5, g=0.8, loss vs second seeded map; Fisher 64 mu_a blocks "
+                                  "(6^3 voxels), 576 detectors"},
+           "terms": st["terms"], "gradient_s": float(np.mean(tg)), "fisher_s": float(np.mean(tf)),
+           "fisher_eig_min_max": [float(ev[0]), float(ev[-1])]}
+    print(json.dumps(res), flush=True)
+    med.close()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -168,6 +218,7 @@ def main():
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--plain-sample", type=int, default=2)
+    ap.add_argument("--config3", action="store_true", help="time the gradient and Fisher workload instead")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="only run 3 plain device copies of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -189,6 +240,8 @@ def main():
 
     if args.serve:
         return serve(args)
+    if args.config3:
+        return config3(args)
     from doi_amd import media, rte
 
     n, L = args.n, args.L
@@ -239,6 +292,12 @@ def main():
     achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
     pairs_per_s = prof["prop_pairs"] / (prof["prop_ms"] * 1e-3) if prof["prop_ms"] > 0 else 0.0
 
+    traffic = None
+    if n == 32 and L == 5 and args.group in (0, 2):
+        # beams of a medium go in groups of two, the odd one alone
+        pairs = (args.beams // 2) * 2
+        per_medium = pairs * TRAFFIC_PER_SOLVE_TERM[2] + (args.beams - pairs) * TRAFFIC_PER_SOLVE_TERM[1]
+        traffic = per_medium * args.media * terms_mean / launches * args.steps
     out = {}
     # error against an all-FP64 solve of medium 0's beams (same term cap)
     if not args.no_check:
@@ -280,7 +339,7 @@ def main():
         "terms_mean": terms_mean,
         "voxel_pair_propagations_per_s": pairs_per_s,
         "roofline": {"bound": "mfma", "pipe": "valu-fp32", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "kernel": "k_prop (cached path)", "avg_launch_ms": avg_ms,
                      "flops_per_launch": flops_per_launch},
     }

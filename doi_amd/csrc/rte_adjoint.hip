@@ -244,16 +244,16 @@ __global__ void k_make_wq(GridDev<T> g, Scales sc, TA gan, const T *mus, const T
 // memory.  grid = (ceil(nblk/4), nchunk), block = 256 (4 bricks).
 // ---------------------------------------------------------------------------
 template <typename T, typename TA, bool LDS>
-__global__ __launch_bounds__(256) void k_backproject(GridDev<T> g, int chunk_len, T thr2, const T *pairbuf,
-                                                     TA scale, TA *gt)
+__global__ __launch_bounds__(1024) void k_backproject(GridDev<T> g, int chunk_len, T thr2, const T *pairbuf,
+                                                      TA scale, TA *gt)
 {
     extern __shared__ unsigned char smem_raw[];
     TA *sacc = reinterpret_cast<TA *>(smem_raw);
     const int lane = threadIdx.x & 63;
-    const int blk = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int blk = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x >> 6);
     const int chunk = blockIdx.y;
     if constexpr (LDS) {
-        for (int i = threadIdx.x; i < g.N; i += 256) sacc[i] = TA(0);
+        for (int i = threadIdx.x; i < g.N; i += blockDim.x) sacc[i] = TA(0);
         __syncthreads();
     }
     if (blk < g.nblk) {
@@ -292,7 +292,7 @@ __global__ __launch_bounds__(256) void k_backproject(GridDev<T> g, int chunk_len
     }
     if constexpr (LDS) {
         __syncthreads();
-        for (int i = threadIdx.x; i < g.N; i += 256)
+        for (int i = threadIdx.x; i < g.N; i += blockDim.x)
             if (sacc[i] != TA(0)) atomicAdd(&gt[i], sacc[i]);
     }
 }
@@ -450,11 +450,28 @@ template <typename T, typename TA>
 hipError_t launch_backproject(const GridDev<T> &g, int nchunk, int chunk_len, T thr2, const T *pairbuf, TA scale,
                               TA *gt, hipStream_t st)
 {
-    dim3 grid((g.nblk + 3) / 4, nchunk), block(256);
+    static int lds_max = -1;
+    if (lds_max < 0) {
+        int dev = 0, v = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+        if (e != hipSuccess) return e;
+        lds_max = v;
+    }
     const size_t lds = (size_t)g.N * sizeof(TA);
-    if (lds <= 64 * 1024) {
+    if (lds <= (size_t)lds_max) {
+        // a large accumulator leaves room for one block per CU: make it 16 waves
+        const int threads = lds > 64 * 1024 ? 1024 : 256;
+        if (lds > 64 * 1024) {
+            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_backproject<T, TA, true>),
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+        }
+        const int bricks = threads / kWave;
+        dim3 grid((g.nblk + bricks - 1) / bricks, nchunk), block(threads);
         hipLaunchKernelGGL((k_backproject<T, TA, true>), grid, block, lds, st, g, chunk_len, thr2, pairbuf, scale, gt);
     } else {
+        dim3 grid((g.nblk + 3) / 4, nchunk), block(256);
         hipLaunchKernelGGL((k_backproject<T, TA, false>), grid, block, 0, st, g, chunk_len, thr2, pairbuf, scale, gt);
     }
     return hipGetLastError();
