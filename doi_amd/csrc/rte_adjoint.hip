@@ -133,31 +133,56 @@ __global__ __launch_bounds__(kWave) void k_adj_combine(AdjCombArgs<T, TA> A)
         const TA sgn = (l_of<L>(a) & 1) ? TA(-1) : TA(1);
         acc[a] = sgn * acc[a] * A.nsc[a];
     }
-    // self block S(mu) = self_scale sum_k c_k M_k, c_k = (-mu)^{k-1}/k!, and its mu-derivative
-    const TA mu = TA(A.mut[v]);
-    TA coef = TA(1), dcoef = TA(0), pw = TA(1);  // pw = (-mu)^{k-1}
+    // self block S(mu) = self_scale sum_k c_k M_k, c_k = (-mu)^{k-1}/k!, and its mu-derivative;
+    // the FP32 path uses the same FP32 entries as the forward combine (S symmetric)
     TA gself = TA(0);
-    TA fact = TA(1);
-    for (int k = 0; k < kNMom; ++k) {
-        // k here is moment index k+1: c = (-mu)^k/(k+1)!, dc/dmu = -k (-mu)^{k-1}/(k+1)!
-        fact *= TA(k + 1);
-        dcoef = (k == 0) ? TA(0) : -TA(k) * pw / fact;
-        pw = (k == 0) ? TA(1) : pw * (-mu);
-        coef = pw / fact;
-        const TA *__restrict__ M = A.mom + (size_t)k * K * K;
-        TA gk = TA(0);
+    if constexpr (sizeof(T) == 4) {
+        T xt[K], wt[K], st[K];
 #pragma unroll
         for (int a = 0; a < K; ++a) {
-            TA sx = TA(0), sw = TA(0);
-#pragma unroll
-            for (int b = 0; b < K; ++b) {
-                sx += M[a * K + b] * xi[b];
-                sw += M[a * K + b] * w[b];
-            }
-            acc[a] += A.self_scale * coef * sx;
-            gk += xi[a] * sw;
+            xt[a] = T(xi[a]);
+            wt[a] = T(w[a]);
+            st[a] = T(0);
         }
-        gself += dcoef * gk;
+        const T mu = A.mut[v];
+        T coef = T(1), dcoef = T(0), pw = T(1), fact = T(1), gs = T(0);
+        for (int k = 0; k < kNMom; ++k) {
+            fact *= T(k + 1);
+            dcoef = (k == 0) ? T(0) : -T(k) * pw / fact;
+            pw = (k == 0) ? T(1) : pw * (-mu);
+            coef = pw / fact;
+            T gk = T(0);
+            self_apply_adj<K>(st, gk, coef, A.momT + (size_t)k * K * K, xt, wt, std::make_integer_sequence<int, K>{});
+            gs += dcoef * gk;
+        }
+#pragma unroll
+        for (int a = 0; a < K; ++a) acc[a] += A.self_scale * TA(st[a]);
+        gself = TA(gs);
+    } else {
+        const TA mu = TA(A.mut[v]);
+        TA coef = TA(1), dcoef = TA(0), pw = TA(1);  // pw = (-mu)^{k-1}
+        TA fact = TA(1);
+        for (int k = 0; k < kNMom; ++k) {
+            // k here is moment index k+1: c = (-mu)^k/(k+1)!, dc/dmu = -k (-mu)^{k-1}/(k+1)!
+            fact *= TA(k + 1);
+            dcoef = (k == 0) ? TA(0) : -TA(k) * pw / fact;
+            pw = (k == 0) ? TA(1) : pw * (-mu);
+            coef = pw / fact;
+            const TA *__restrict__ M = A.mom + (size_t)k * K * K;
+            TA gk = TA(0);
+#pragma unroll
+            for (int a = 0; a < K; ++a) {
+                TA sx = TA(0), sw = TA(0);
+#pragma unroll
+                for (int b = 0; b < K; ++b) {
+                    sx += M[a * K + b] * xi[b];
+                    sw += M[a * K + b] * w[b];
+                }
+                acc[a] += A.self_scale * coef * sx;
+                gk += xi[a] * sw;
+            }
+            gself += dcoef * gk;
+        }
     }
     A.gt[v] += A.self_scale * A.dV * gself;
 #pragma unroll

@@ -3,6 +3,8 @@
 
 #include "rte_internal.hpp"
 
+#include <utility>
+
 namespace rte {
 
 // ---------------------------------------------------------------------------
@@ -112,7 +114,8 @@ __device__ __forceinline__ void pair_geom(PairGeom<T, L> &G, T x, T y, T z, cons
 template <typename T, int L>
 __device__ __forceinline__ T pair_dot(const T (&w)[(L + 1) * (L + 1)], const PairGeom<T, L> &G)
 {
-    T p = w[0];
+    // two partial sums: halves the dependent chain of the dot product
+    T p = w[0], q = T(0);
 #pragma unroll
     for (int l = 1; l <= L; ++l) p += G.P[l][0] * w[l * l + l];
 #pragma unroll
@@ -123,9 +126,10 @@ __device__ __forceinline__ T pair_dot(const T (&w)[(L + 1) * (L + 1)], const Pai
             A += G.P[l][m] * w[l * l + l + m];
             B += G.P[l][m] * w[l * l + l - m];
         }
-        p += G.C[m] * A + G.S[m] * B;
+        q += G.C[m] * A;
+        q += G.S[m] * B;
     }
-    return p;
+    return p + q;
 }
 
 // acc' += t Rt(u)
@@ -195,6 +199,56 @@ template <typename T>
 __device__ __forceinline__ cptr<T> as_const(const T *p) { return (cptr<T>)(p); }
 template <typename T>
 __device__ __forceinline__ gptr<T> as_global(const T *p) { return (gptr<T>)(p); }
+
+// Whether the self-block moments M_k[a][b] = sum_w R_a R_b beta^k sin(theta)
+// can couple real-basis coefficients a and b.  Reflecting the voxel in x, y
+// or z multiplies R_lm by (-1)^|m| (x; once more -1 for the sin type), by +1
+// for the cos type and -1 for the sin type (y), and by (-1)^(l+|m|) (z); the
+// angular grid and beta share those symmetries, so an entry survives only if
+// R_a R_b is even under all three: the same type, |m| and l of equal parity.
+// (With beta in the reference's float geometry the others are ~1e-7 of the
+// largest entry rather than 0: the FP32 path skips them, FP64 keeps them.)
+__host__ __device__ constexpr bool self_couples(int a, int b)
+{
+    int la = 0, lb = 0;
+    while ((la + 1) * (la + 1) <= a) ++la;
+    while ((lb + 1) * (lb + 1) <= b) ++lb;
+    const int ma = a - la * la - la, mb = b - lb * lb - lb;
+    const bool sin_a = ma < 0, sin_b = mb < 0;
+    const int am = ma < 0 ? -ma : ma, bm = mb < 0 ? -mb : mb;
+    return sin_a == sin_b && ((am - bm) & 1) == 0 && ((la - lb) & 1) == 0;
+}
+
+// s += M[a][b] w[b] over the b that self_couples(a, b) keeps, and the same for
+// a second vector; unrolled at compile time, so the skipped entries cost nothing
+template <int a, int b, int K, typename T>
+__device__ __forceinline__ void self_term(T &s, const T *__restrict__ M, const T (&w)[K])
+{
+    if constexpr (self_couples(a, b)) s += M[a * K + b] * w[b];
+}
+template <int a, int K, typename T, int... bs>
+__device__ __forceinline__ T self_row(const T *__restrict__ M, const T (&w)[K], std::integer_sequence<int, bs...>)
+{
+    T s = T(0);
+    (self_term<a, bs, K>(s, M, w), ...);
+    return s;
+}
+// st[a] += coef * sum_b M[a][b] w[b] for every a (the kept entries)
+template <int K, typename T, int... as>
+__device__ __forceinline__ void self_apply(T (&st)[K], T coef, const T *__restrict__ M, const T (&w)[K],
+                                           std::integer_sequence<int, as...>)
+{
+    ((st[as] += coef * self_row<as, K>(M, w, std::make_integer_sequence<int, K>{})), ...);
+}
+// as self_apply, and gk += sum_a x[a] (M w)[a] (the mu-derivative term of the adjoint)
+template <int K, typename T, int... as>
+__device__ __forceinline__ void self_apply_adj(T (&st)[K], T &gk, T coef, const T *__restrict__ M,
+                                               const T (&x)[K], const T (&w)[K], std::integer_sequence<int, as...>)
+{
+    ((st[as] += coef * self_row<as, K>(M, x, std::make_integer_sequence<int, K>{}),
+      gk += x[as] * self_row<as, K>(M, w, std::make_integer_sequence<int, K>{})),
+     ...);
+}
 
 // destination of a lane: brick blk, lane -> (di, dj, dk)
 template <typename T>

@@ -843,6 +843,7 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
     ca.nchunk = nch;
     ca.op = 0;
     ca.mom = sizeof(TA) == 4 ? (const TA *)mom->mf.p : (const TA *)mom->md.p;
+    ca.momT = sizeof(T) == 4 ? (const T *)mom->mf.p : (const T *)mom->md.p;
     ca.self_scale = TA((M_PI / kAngRes) * (2.0 * M_PI / kAngRes) / (dx * dy * dz));
     ca.dV = TA(dx * dy * dz);
     {
@@ -1141,6 +1142,7 @@ struct AdjointRun {
         ca.g = g;
         ca.nchunk = nch;
         ca.mom = sizeof(TA) == 4 ? (const TA *)mom->mf.p : (const TA *)mom->md.p;
+        ca.momT = sizeof(T) == 4 ? (const T *)mom->mf.p : (const T *)mom->md.p;
         ca.self_scale = TA((M_PI / kAngRes) * (2.0 * M_PI / kAngRes) / dV);
         ca.dV = TA(dV);
         double Q[kMaxL + 1];
@@ -1225,6 +1227,7 @@ struct AdjointRun {
         aa.g = g;
         aa.nchunk = nch;
         aa.mom = ca.mom;
+        aa.momT = ca.momT;
         aa.self_scale = ca.self_scale;
         aa.dV = ca.dV;
         for (int l = 0; l <= kMaxL; ++l) aa.coefE[l] = ca.coefE[l];
@@ -1602,6 +1605,7 @@ static int propagate_impl(rte_ctx *ctx, rte_medium *m, int L, double prop_thresh
     ca.nchunk = nch;
     ca.op = 1;
     ca.mom = sizeof(TA) == 4 ? (const TA *)mom->mf.p : (const TA *)mom->md.p;
+    ca.momT = sizeof(T) == 4 ? (const T *)mom->mf.p : (const T *)mom->md.p;
     ca.self_scale = TA((M_PI / kAngRes) * (2.0 * M_PI / kAngRes) / (dx * dy * dz));
     ca.dV = TA(dx * dy * dz);
     ca.aw_out = dAW.as<TA>();
@@ -1940,6 +1944,7 @@ struct Bucket : BucketBase {
         ca.nchunk = nch;
         ca.op = 0;
         ca.mom = sizeof(TA) == 4 ? (const TA *)mom->mf.p : (const TA *)mom->md.p;
+        ca.momT = sizeof(T) == 4 ? (const T *)mom->mf.p : (const T *)mom->md.p;
         ca.self_scale = TA((M_PI / kAngRes) * (2.0 * M_PI / kAngRes) / (dx * dy * dz));
         ca.dV = TA(dx * dy * dz);
         double Q[kMaxL + 1];

@@ -72,6 +72,7 @@ struct CombArgs {
     int nchunk;
     int op;
     const TA *mom;        // [kNMom][K][K] self-block moments
+    const T *momT;        // the same in the compute precision (FP32 self block)
     TA self_scale;        // dtheta dphi / dV (S's 1/dV; the result is scaled by dV)
     TA dV;
     TA coefE[kMaxL + 1];  // exitance weight of w_l0
@@ -130,6 +131,7 @@ struct AdjCombArgs {
     GridDev<T> g;
     int nchunk;
     const TA *mom;        // [kNMom][K][K]
+    const T *momT;        // the same in the compute precision
     TA self_scale;        // dtheta dphi / dV
     TA dV;
     TA coefE[kMaxL + 1];

@@ -236,17 +236,35 @@ __global__ __launch_bounds__(kWave) void k_combine(CombArgs<T, TA> A, const Solv
 #pragma unroll
     for (int a = 0; a < K; ++a) acc[a] *= A.nsc[a];
     // self block: sum_k coef_k (M_k w), coef_k = (-mu)^{k-1}/k!
-    const TA mu = TA(S.mut[v]);
-    TA coef = TA(1);
-    for (int k = 0; k < kNMom; ++k) {
-        if (k > 0) coef = coef * (-mu) / TA(k + 1);
-        const TA *__restrict__ M = A.mom + (size_t)k * K * K;
+    if constexpr (sizeof(T) == 4) {
+        // in FP32, on the entries the voxel's reflection symmetries leave
+        T wt[K], st[K];
 #pragma unroll
         for (int a = 0; a < K; ++a) {
-            TA s = TA(0);
+            wt[a] = T(w[a]);
+            st[a] = T(0);
+        }
+        const T mu = S.mut[v];
+        T coef = T(1);
+        for (int k = 0; k < kNMom; ++k) {
+            if (k > 0) coef = coef * (-mu) / T(k + 1);
+            self_apply<K>(st, coef, A.momT + (size_t)k * K * K, wt, std::make_integer_sequence<int, K>{});
+        }
 #pragma unroll
-            for (int b = 0; b < K; ++b) s += M[a * K + b] * w[b];
-            acc[a] += A.self_scale * coef * s;
+        for (int a = 0; a < K; ++a) acc[a] += A.self_scale * TA(st[a]);
+    } else {
+        const TA mu = TA(S.mut[v]);
+        TA coef = TA(1);
+        for (int k = 0; k < kNMom; ++k) {
+            if (k > 0) coef = coef * (-mu) / TA(k + 1);
+            const TA *__restrict__ M = A.mom + (size_t)k * K * K;
+#pragma unroll
+            for (int a = 0; a < K; ++a) {
+                TA s = TA(0);
+#pragma unroll
+                for (int b = 0; b < K; ++b) s += M[a * K + b] * w[b];
+                acc[a] += A.self_scale * coef * s;
+            }
         }
     }
 #pragma unroll
