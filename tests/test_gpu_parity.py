@@ -23,7 +23,7 @@ def _field(K, N, seed):
 
 
 @pytest.mark.parametrize("shape,L", [((4, 4, 4), 0), ((4, 4, 4), 3), ((5, 3, 4), 2), ((4, 5, 6), 5),
-                                     ((6, 6, 6), 1)])
+                                     ((6, 6, 6), 1), ((4, 4, 3), 6), ((3, 4, 4), 7)])
 @pytest.mark.parametrize("prec", [rte.PREC_FP64, rte.PREC_MIXED, rte.PREC_FP32])
 @pytest.mark.parametrize("path", [rte.PATH_CACHED, rte.PATH_MARCH])
 def test_propagate_heterogeneous(ctx, shape, L, prec, path):
@@ -149,17 +149,61 @@ def test_batch_equals_alone(ctx):
 
 
 def test_beams_sharing_a_medium_equal_alone(ctx):
-    """Seven beams on one medium (propagated as groups of 5 and 2 right-hand
-    sides) give the same bits as each beam solved alone."""
+    """Seven beams on one medium, propagated as groups of 5 and 2 (group size 5)
+    or 2, 2, 2 and 1 (group size 2) right-hand sides, give the same bits as each
+    beam solved alone."""
     shape, L = (12, 12, 12), 5
     bounds = bounds_for(shape)
     m = ctx.medium(shape, bounds, *hetero_maps(shape, seed=31), 0.9)
     xs = (np.arange(12) + 0.5) * 0.1 - 0.6
     beams = [(xs[i], xs[(3 * i + 2) % 12], 0.1, 1.0) for i in range(7)]
     p = rte.make_params(L, precision=rte.PREC_MIXED, stop_rule=rte.STOP_DELTA, tol=1e-6, max_terms=20)
-    batch, st = ctx.solve_media([m] * 7, beams, p)
-    for i, b in enumerate(beams):
-        alone, st1 = ctx.solve_media([m], [b], p)
-        assert st1[0]["terms"] == st[i]["terms"]
-        assert np.array_equal(alone[0], batch[i])
+    alone = [ctx.solve_media([m], [b], p) for b in beams]
+    try:
+        for gsize in (5, 2):
+            ctx.set_group_size(gsize)
+            batch, st = ctx.solve_media([m] * 7, beams, p)
+            for i in range(7):
+                assert alone[i][1][0]["terms"] == st[i]["terms"]
+                assert np.array_equal(alone[i][0][0], batch[i]), (gsize, i)
+    finally:
+        ctx.set_group_size(2)
     m.close()
+
+
+def test_solves_on_different_media_equal_alone(ctx):
+    """Five media with one beam each in one batch, with group sizes 2 and 1:
+    every result has the same bits as the solve alone."""
+    shape, L = (12, 12, 12), 5
+    bounds = bounds_for(shape)
+    mm = [ctx.medium(shape, bounds, *hetero_maps(shape, seed=40 + k), 0.85) for k in range(5)]
+    xs = (np.arange(12) + 0.5) * 0.1 - 0.6
+    beams = [(xs[2 * k + 1], xs[11 - 2 * k], 0.1, 1.0) for k in range(5)]
+    p = rte.make_params(L, precision=rte.PREC_MIXED, stop_rule=rte.STOP_DELTA, tol=1e-6, max_terms=25)
+    alone = [ctx.solve_media([m], [b], p)[0][0] for m, b in zip(mm, beams)]
+    batch, _ = ctx.solve_media(mm, beams, p)
+    try:
+        ctx.set_group_size(1)
+        unpaired, _ = ctx.solve_media(mm, beams, p)
+    finally:
+        ctx.set_group_size(2)
+    for k in range(5):
+        assert np.array_equal(batch[k], alone[k]), k
+        assert np.array_equal(unpaired[k], alone[k]), k
+    for m in mm:
+        m.close()
+
+
+@pytest.mark.parametrize("L", [6, 7])
+def test_high_order_solve(ctx, L):
+    """The two highest supported orders end to end against the oracle (FP64 and mixed)."""
+    shape = (6, 6, 5)
+    bounds = bounds_for(shape)
+    ma, ms = hetero_maps(shape, seed=60 + L)
+    beam = (0.05, -0.05, 0.1, 1.0)
+    ref = O.Problem(shape, bounds, ma, ms, 0.85).solve(beam, L, stop_rule=O.STOP_DELTA, tol=1e-6, max_terms=20)
+    for prec, tol in ((rte.PREC_FP64, 1e-9), (rte.PREC_MIXED, 1e-5)):
+        p = rte.make_params(L, precision=prec, stop_rule=rte.STOP_DELTA, tol=1e-6, max_terms=20)
+        img, st = ctx.solve(shape, bounds, ma, ms, 0.85, beam, p)
+        assert st["terms"] == ref["terms"]
+        assert rel_max(img, ref["exitance"]) < tol
