@@ -1,0 +1,152 @@
+"""Reference file formats (doi_amd.refio): the INI rules of the iniparser the
+reference links, the tissue-map and OutputImage formats, and parameter files
+written the way the reference CPU package was run for the golden fixtures:
+loaded into a problem and solved by the oracle they give the reference
+binary's images (CPU only)."""
+import struct
+
+import numpy as np
+import pytest
+
+from doi_amd import refio
+from oracle import oracle as O
+from tests.helpers import rel_max
+
+INI = """[Runtime]
+TransFile = trans.out
+ReflectFile = refl.out
+
+[Algorithm]
+nL = {L}
+nTerms = 30
+
+[Geometry]
+nX = {nx}
+nY = {ny}
+nZ = {nz}
+xMin = {b[0]!r}
+xMax = {b[1]!r}
+yMin = {b[2]!r}
+yMax = {b[3]!r}
+zMin = {b[4]!r}
+zMax = {b[5]!r}
+propThresh = {pt!r}
+selfSubVox = 1
+
+[Phantom]
+mua = {mua!r}
+mus = {mus!r}
+g = {g!r}
+n = {n!r}
+
+[Source]
+posX = {beam[0]!r}
+posY = {beam[1]!r}
+diam = {beam[2]!r}
+magnitude = 1.0
+modulation = 0.0
+"""
+
+
+def ini_for(case):
+    nx, ny, nz = case["shape"]
+    b = case["beam"]
+    return INI.format(L=case["L"], nx=nx, ny=ny, nz=nz, b=case["bounds"], pt=case["prop_thresh"], mua=case["mua"],
+                      mus=case["mus"], g=case["g"], n=case["n"], beam=(b["pos_x"], b["pos_y"], b["diam"]))
+
+
+def test_ini_rules():
+    d = refio.parse_ini("""
+# comment
+; another
+[Geometry]
+NX = 8 ; inline comment
+nY=0x10
+nZ = 7.9
+xMin = -0.4abc
+Name = "quoted ; not a comment"
+Other = 'single'
+long = 1, \\
+  2
+[phantom]
+mua = 0.01
+mua = 0.02
+""")
+    assert refio.get_int(d, "Geometry:nX", -1) == 8
+    assert refio.get_int(d, "geometry:NY", -1) == 16
+    assert refio.get_int(d, "Geometry:nZ", -1) == 7
+    assert refio.get_double(d, "Geometry:xMin", 0.0) == -0.4
+    assert refio.get_string(d, "Geometry:name") == "quoted ; not a comment"
+    assert refio.get_string(d, "Geometry:other") == "single"
+    assert refio.get_string(d, "Geometry:long") == "1,2"
+    assert refio.get_double(d, "Phantom:mua", 0.0) == 0.02
+    assert refio.get_double(d, "Phantom:missing", 5.5) == 5.5
+
+
+def test_image_round_trip_and_layout(tmp_path):
+    real = np.arange(12, dtype=np.float64).reshape(3, 4) * 0.5
+    p = tmp_path / "img.out"
+    refio.write_image(str(p), real)
+    raw = p.read_bytes()
+    assert len(raw) == 8 + 2 * 12 * 8
+    assert struct.unpack("<ii", raw[:8]) == (4, 3)
+    assert struct.unpack("<d", raw[8 + 8:8 + 16])[0] == 0.5   # x fastest
+    re_, im = refio.read_image(str(p))
+    assert np.array_equal(re_, real) and not np.any(im)
+
+
+def test_tissue_map_problem(tmp_path):
+    shape = (4, 3, 5)
+    N = 60
+    tiss = (np.arange(N) % 3 + 1).astype(np.uint8)
+    tiss.tofile(str(tmp_path / "map.bin"))
+    text = """[Algorithm]
+nL = 2
+[Geometry]
+nX = 4
+nY = 3
+nZ = 5
+xMin = -0.2
+xMax = 0.2
+yMin = -0.15
+yMax = 0.15
+zMin = 0
+zMax = 0.5
+[Phantom]
+g = 0.8
+n = 1.0
+NoTiss = 3
+TissAbs1 = 0.01
+TissSc1 = 1.0
+TissAbs2 = 0.02
+TissSc2 = 1.5
+TissAbs3 = 0.05
+TissMapFile = map.bin
+[Source]
+posX = 0.05
+"""
+    (tmp_path / "p.ini").write_text(text)
+    prob = refio.load(str(tmp_path / "p.ini"))
+    assert prob.shape == shape and prob.L == 2 and prob.bounds == (-0.2, 0.2, -0.15, 0.15, 0.0, 0.5)
+    mua = np.array([0.0, 0.01, 0.02, 0.05])[tiss]
+    mus = np.array([0.0, 1.0, 1.5, 1.0])[tiss]          # TissSc3 defaults to 1.0
+    assert np.array_equal(prob.mu_a, mua) and np.array_equal(prob.mu_s, mus)
+    assert prob.beam == (0.05, 0.0, 0.2, 1.0)             # reference defaults
+    bad = tiss.copy()
+    bad[7] = 0
+    bad.tofile(str(tmp_path / "map.bin"))
+    with pytest.raises(ValueError):
+        refio.load(str(tmp_path / "p.ini"))
+
+
+def test_parameter_files_reproduce_reference_images(tmp_path, golden_ref):
+    """The golden cases' parameter files through refio and the oracle give the
+    reference binary's images and term counts."""
+    for case in golden_ref[:4] + golden_ref[6:]:
+        p = tmp_path / (case["name"] + ".ini")
+        p.write_text(ini_for(case))
+        prob = refio.load(str(p))
+        P = O.Problem(prob.shape, prob.bounds, prob.mu_a, prob.mu_s, prob.g, n=prob.n, c=prob.c)
+        r = P.solve(prob.beam, prob.L, stop_rule=O.STOP_LAMBDA, max_terms=1000, prop_thresh=prob.prop_thresh)
+        assert r["terms"] == case["terms"], case["name"]
+        assert rel_max(r["exitance"], np.array(case["trans_image"])) < 1e-12, case["name"]
