@@ -412,6 +412,10 @@ __global__ __launch_bounds__(kWave) void k_ref_source(GridDev<T> g, Scales sc, c
         S.dE[pix] = TA(0);
         S.bsum[pix] = bs;
     }
+    if (v < P) {
+        S.R[v] = TA(0);
+        S.dR[v] = TA(0);
+    }
 }
 
 // y += x over n elements

@@ -163,6 +163,53 @@ static double legendre(int l, double x)
 
 // Q_l = int_0^1 P_l(u) u du: the reference's 100-point left Riemann sum
 // (CPU package/src/Dist.c:190-193,211-218) or exact (Gauss-Legendre, 32 nodes)
+static void quad_weights(int L, int quad, double *Q);
+
+// per-solve pixel buffers: dE, bottom sums, E, dR, R
+constexpr int kPixBufs = 5;
+template <typename T, typename TA>
+static void set_pixels(SolveDev<T, TA> &s, TA *base, int P)
+{
+    s.dE = base;
+    s.bsum = base + P;
+    s.E = base + 2 * P;
+    s.dR = base + 3 * P;
+    s.R = base + 4 * P;
+}
+
+// R_l = int_{-1}^{0} P_l(u) u du: the reference's left Riemann sum over u = p/100 - 1
+// (CPU package/src/Dist.c:146-160), or exactly (-1)^(l+1) int_0^1 P_l(u) u du
+static void quad_weights_refl(int L, int quad, double *R)
+{
+    if (quad == RTE_QUAD_REF) {
+        for (int l = 0; l <= L; ++l) {
+            double s = 0.0;
+            for (int p = 0; p < 100; ++p) {
+                const double u = (double)p / 100.0 - 1.0;
+                s += legendre(l, u) * u / 100.0;
+            }
+            R[l] = s;
+        }
+        return;
+    }
+    quad_weights(L, quad, R);
+    for (int l = 0; l <= L; ++l) R[l] *= (l & 1) ? 1.0 : -1.0;
+}
+
+// weights of w_l0 in the bottom exitance (CPU package/src/Dist.c:211-218) and
+// the top reflectance (Dist.c:154-158, with its minus sign), in c_m-free units
+static void face_coefs(int L, int quad, double dx, double dy, double *cE, double *cR)
+{
+    double Q[kMaxL + 1], R[kMaxL + 1];
+    quad_weights(L, quad, Q);
+    quad_weights_refl(L, quad, R);
+    for (int l = 0; l <= kMaxL; ++l) {
+        const double f = l <= L ? 2.0 * M_PI * dx * dy / std::sqrt(4.0 * M_PI) * std::sqrt(2.0 * l + 1.0) : 0.0;
+        cE[l] = l <= L ? f * Q[l] : 0.0;
+        cR[l] = l <= L ? -f * R[l] : 0.0;
+    }
+}
+
 static void quad_weights(int L, int quad, double *Q)
 {
     if (quad == RTE_QUAD_REF) {
@@ -710,7 +757,8 @@ struct RefSrc {
 
 template <typename T, typename TA>
 static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_beam *beams,
-                     const rte_params *p, double *exitance, rte_stats *stats, const RefSrc<TA> *rs = nullptr)
+                     const rte_params *p, double *exitance, rte_stats *stats, const RefSrc<TA> *rs = nullptr,
+                     double *reflectance = nullptr)
 {
     const rte_grid &grid = media[0]->grid;
     const int L = p->L, K = (L + 1) * (L + 1);
@@ -752,7 +800,7 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
     const size_t szW = (size_t)N * K, szPart = (size_t)nch * N * K;
     HIP_TRY(ctx->ws_W.alloc(szW * sizeof(T) * nb));
     HIP_TRY(ctx->ws_part.alloc(szPart * sizeof(T) * nb));
-    HIP_TRY(ctx->ws_pix.alloc((size_t)3 * P * sizeof(TA) * nb));
+    HIP_TRY(ctx->ws_pix.alloc((size_t)kPixBufs * P * sizeof(TA) * nb));
     HIP_TRY(ctx->ws_red.alloc((size_t)2 * sizeof(TA) * nb));
     HIP_TRY(ctx->ws_sv.alloc(sizeof(SolveDev<T, TA>) * nb * 2));
     HIP_TRY(ctx->ws_beams.alloc(sizeof(BeamDev) * nb));
@@ -769,10 +817,7 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
         s.mut = sizeof(T) == 4 ? (const T *)m->mut_f.p : (const T *)m->mut_d.p;
         s.mus = sizeof(T) == 4 ? (const T *)m->mus_f.p : (const T *)m->mus_d.p;
         s.g = T(m->g);
-        TA *pix = ctx->ws_pix.as<TA>() + (size_t)3 * P * i;
-        s.dE = pix;
-        s.bsum = pix + P;
-        s.E = pix + 2 * P;
+        set_pixels(s, ctx->ws_pix.as<TA>() + (size_t)kPixBufs * P * i, P);
         s.red = ctx->ws_red.as<TA>() + 2 * i;
         s.mode = 1;
         hs[i] = HostSolve{m, 0, 0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0};
@@ -847,10 +892,12 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
     ca.self_scale = TA((M_PI / kAngRes) * (2.0 * M_PI / kAngRes) / (dx * dy * dz));
     ca.dV = TA(dx * dy * dz);
     {
-        double Q[kMaxL + 1];
-        quad_weights(L, p->exit_quad, Q);
-        for (int l = 0; l <= L; ++l)
-            ca.coefE[l] = TA(2.0 * M_PI * dx * dy / std::sqrt(4.0 * M_PI) * std::sqrt(2.0 * l + 1.0) * Q[l]);
+        double cE[kMaxL + 1], cR[kMaxL + 1];
+        face_coefs(L, p->exit_quad, dx, dy, cE, cR);
+        for (int l = 0; l <= L; ++l) {
+            ca.coefE[l] = TA(cE[l]);
+            ca.coefR[l] = TA(cR[l]);
+        }
         for (int a = 0; a < K; ++a) ca.wsum[a] = TA(stop_weight(a));
         for (int a = 0; a < kMaxK; ++a) ca.nsc[a] = TA(sc.n[a]);
     }
@@ -935,6 +982,11 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
         HIP_TRY(hipMemcpyAsync(img.data(), sv[i].E, sizeof(TA) * P, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         for (int q = 0; q < P; ++q) exitance[(size_t)i * P + q] = (double)img[q];
+        if (reflectance) {
+            HIP_TRY(hipMemcpyAsync(img.data(), sv[i].R, sizeof(TA) * P, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            for (int q = 0; q < P; ++q) reflectance[(size_t)i * P + q] = (double)img[q];
+        }
     }
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -975,8 +1027,8 @@ static int check_params(const rte_params *p)
     return RTE_OK;
 }
 
-extern "C" int rte_solve_media(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_beam *beams,
-                               const rte_params *params, double *exitance, rte_stats *stats)
+extern "C" int rte_solve_images(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_beam *beams,
+                                const rte_params *params, double *exitance, double *reflectance, rte_stats *stats)
 {
     if (!ctx || !media || !beams || !exitance) return fail(RTE_E_ARG, "rte_solve_media: NULL argument");
     if (nb < 1) return fail(RTE_E_ARG, "rte_solve_media: nb < 1");
@@ -994,10 +1046,19 @@ extern "C" int rte_solve_media(rte_ctx *ctx, int nb, rte_medium *const *media, c
     }
     if ((rc = set_device(ctx))) return rc;
     switch (params->precision) {
-    case RTE_PREC_FP64: return run_batch<double, double>(ctx, nb, media, beams, params, exitance, stats);
-    case RTE_PREC_MIXED: return run_batch<float, double>(ctx, nb, media, beams, params, exitance, stats);
-    default: return run_batch<float, float>(ctx, nb, media, beams, params, exitance, stats);
+    case RTE_PREC_FP64:
+        return run_batch<double, double>(ctx, nb, media, beams, params, exitance, stats, nullptr, reflectance);
+    case RTE_PREC_MIXED:
+        return run_batch<float, double>(ctx, nb, media, beams, params, exitance, stats, nullptr, reflectance);
+    default:
+        return run_batch<float, float>(ctx, nb, media, beams, params, exitance, stats, nullptr, reflectance);
     }
+}
+
+extern "C" int rte_solve_media(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_beam *beams,
+                               const rte_params *params, double *exitance, rte_stats *stats)
+{
+    return rte_solve_images(ctx, nb, media, beams, params, exitance, nullptr, stats);
 }
 
 extern "C" int rte_solve(rte_ctx *ctx, const rte_grid *grid, const rte_optics *optics,
@@ -1024,7 +1085,7 @@ extern "C" long long rte_solve_footprint(const rte_grid *grid, const rte_params 
     const int nch = pick_chunks(g.nblk, 1, g.N);
     long long bytes = N * 8 * 3 + N * 4 * 2;             // maps
     bytes += N * K * es * (1 + nch);                      // W + partials
-    bytes += (long long)grid->nx * grid->ny * 3 * 8 + 64; // pixel buffers
+    bytes += (long long)grid->nx * grid->ny * kPixBufs * 8 + 64; // pixel buffers
     if (path_mode == RTE_PATH_CACHED) bytes += N * (long long)g.npad * es;
     return bytes;
 }
@@ -1086,7 +1147,7 @@ struct AdjointRun {
         HIP_TRY(dW.alloc(F * sizeof(T)));
         HIP_TRY(dS0.alloc(F * sizeof(T)));
         HIP_TRY(dPart.alloc((size_t)nch * F * sizeof(T)));
-        HIP_TRY(dPix.alloc((size_t)3 * P * sizeof(TA)));
+        HIP_TRY(dPix.alloc((size_t)kPixBufs * P * sizeof(TA)));
         HIP_TRY(dRed.alloc(2 * sizeof(TA)));
         HIP_TRY(dSv.alloc(sizeof(SolveDev<T, TA>)));
         HIP_TRY(dBeam.alloc(sizeof(BeamDev)));
@@ -1109,9 +1170,7 @@ struct AdjointRun {
         sv.mut = sizeof(T) == 4 ? (const T *)m->mut_f.p : (const T *)m->mut_d.p;
         sv.mus = sizeof(T) == 4 ? (const T *)m->mus_f.p : (const T *)m->mus_d.p;
         sv.g = T(m->g);
-        sv.dE = dPix.as<TA>();
-        sv.bsum = sv.dE + P;
-        sv.E = sv.dE + 2 * P;
+        set_pixels(sv, dPix.as<TA>(), P);
         sv.red = dRed.as<TA>();
         sv.mode = 1;
         SolveDev<T, TA> *svd = dSv.as<SolveDev<T, TA>>();
@@ -1145,11 +1204,11 @@ struct AdjointRun {
         ca.momT = sizeof(T) == 4 ? (const T *)mom->mf.p : (const T *)mom->md.p;
         ca.self_scale = TA((M_PI / kAngRes) * (2.0 * M_PI / kAngRes) / dV);
         ca.dV = TA(dV);
-        double Q[kMaxL + 1];
-        quad_weights(L, p->exit_quad, Q);
+        double cR[kMaxL + 1];
+        face_coefs(L, p->exit_quad, dx, dy, coefE, cR);
         for (int l = 0; l <= L; ++l) {
-            coefE[l] = 2.0 * M_PI * dx * dy / std::sqrt(4.0 * M_PI) * std::sqrt(2.0 * l + 1.0) * Q[l];
             ca.coefE[l] = TA(coefE[l]);
+            ca.coefR[l] = TA(cR[l]);
         }
         for (int a = 0; a < K; ++a) ca.wsum[a] = TA(stop_weight(a));
         for (int a = 0; a < kMaxK; ++a) ca.nsc[a] = TA(sc.n[a]);
@@ -1691,7 +1750,7 @@ extern "C" int rte_source(rte_ctx *ctx, const rte_grid *grid, const rte_optics *
         hipStream_t st = ctx->stream;
         DevBuf dW, dpix, dred, dsv, dbeam, dout;
         HIP_TRY(dW.alloc((size_t)K * N * 8));
-        HIP_TRY(dpix.alloc((size_t)3 * P * 8));
+        HIP_TRY(dpix.alloc((size_t)kPixBufs * P * 8));
         HIP_TRY(dred.alloc(16));
         HIP_TRY(dsv.alloc(sizeof(SolveDev<double, double>)));
         HIP_TRY(dbeam.alloc(sizeof(BeamDev)));
@@ -1701,9 +1760,7 @@ extern "C" int rte_source(rte_ctx *ctx, const rte_grid *grid, const rte_optics *
         std::memset(&s, 0, sizeof s);
         s.W = dW.as<double>();
         s.g = m->g;
-        s.dE = dpix.as<double>();
-        s.bsum = s.dE + P;
-        s.E = s.dE + 2 * P;
+        set_pixels(s, dpix.as<double>(), P);
         s.red = dred.as<double>();
         BeamDev b{beam->pos_x, beam->pos_y, half, beam->mag / cols, m->mua_d.as<double>(), m->mus_d.as<double>()};
         HIP_TRY(hipMemcpyAsync(dsv.p, &s, sizeof s, hipMemcpyHostToDevice, st));
@@ -1947,10 +2004,12 @@ struct Bucket : BucketBase {
         ca.momT = sizeof(T) == 4 ? (const T *)mom->mf.p : (const T *)mom->md.p;
         ca.self_scale = TA((M_PI / kAngRes) * (2.0 * M_PI / kAngRes) / (dx * dy * dz));
         ca.dV = TA(dx * dy * dz);
-        double Q[kMaxL + 1];
-        quad_weights(L, p.exit_quad, Q);
-        for (int l = 0; l <= L; ++l)
-            ca.coefE[l] = TA(2.0 * M_PI * dx * dy / std::sqrt(4.0 * M_PI) * std::sqrt(2.0 * l + 1.0) * Q[l]);
+        double cE[kMaxL + 1], cR[kMaxL + 1];
+        face_coefs(L, p.exit_quad, dx, dy, cE, cR);
+        for (int l = 0; l <= L; ++l) {
+            ca.coefE[l] = TA(cE[l]);
+            ca.coefR[l] = TA(cR[l]);
+        }
         for (int a = 0; a < K; ++a) ca.wsum[a] = TA(stop_weight(a));
         for (int a = 0; a < kMaxK; ++a) ca.nsc[a] = TA(sc.n[a]);
         sv.resize(cap);
@@ -2004,7 +2063,7 @@ struct Bucket : BucketBase {
         if (s.mua.alloc(n * 8) || s.mus.alloc(n * 8) || s.mutd.alloc(n * 8) || s.mutf.alloc(n * 4) ||
             s.musf.alloc(n * 4) || (cached && s.ktab.alloc((size_t)N * g.npad * sizeof(T))) ||
             s.W.alloc(F * sizeof(T)) || s.part.alloc((size_t)nch * F * sizeof(T)) ||
-            s.pix.alloc((size_t)3 * P * sizeof(TA))) {
+            s.pix.alloc((size_t)kPixBufs * P * sizeof(TA))) {
             s.free_memory();
             return fail(RTE_E_NOMEM, "server: device memory for a %dx%dx%d L=%d solve", grid.nx, grid.ny, grid.nz, L);
         }
@@ -2035,9 +2094,7 @@ struct Bucket : BucketBase {
         v.mut = mut;
         v.mus = sizeof(T) == 4 ? (const T *)s.musf.p : (const T *)s.mus.p;
         v.g = T(r->g);
-        v.dE = s.pix.template as<TA>();
-        v.bsum = v.dE + P;
-        v.E = v.dE + 2 * P;
+        set_pixels(v, s.pix.template as<TA>(), P);
         v.red = d_red.template as<TA>() + 2 * s.idx;
         v.mode = 1;
         h_init()[k] = v;

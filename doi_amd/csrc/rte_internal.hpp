@@ -51,6 +51,8 @@ struct SolveDev {
     TA *dE;            // [nx*ny] exitance increment of this term
     TA *bsum;          // [nx*ny] per-pixel bottom sums of the next term (stop rule)
     TA *E;             // [nx*ny] accumulated exitance (ballistic included)
+    TA *dR;            // [nx*ny] top-face reflectance increment of this term
+    TA *R;             // [nx*ny] accumulated reflectance
     TA *red;           // [2] reduced (sum dE, sum bsum) of this term
     TA *aw_store;      // [N][K] if set, k_combine also writes A W (unscaled) here
     int mode;          // 0 term, 1 init, 2 final, 3 recompute (see k_bottom / k_combine)
@@ -75,7 +77,8 @@ struct CombArgs {
     const T *momT;        // the same in the compute precision (FP32 self block)
     TA self_scale;        // dtheta dphi / dV (S's 1/dV; the result is scaled by dV)
     TA dV;
-    TA coefE[kMaxL + 1];  // exitance weight of w_l0
+    TA coefE[kMaxL + 1];  // exitance weight of w_l0 (bottom face)
+    TA coefR[kMaxL + 1];  // reflectance weight of w_l0 (top face)
     TA wsum[kMaxK];       // stop-rule weight of coefficient a
     TA nsc[kMaxK];        // n_a of the scaled coefficients
     TA *aw_out;           // op = 1

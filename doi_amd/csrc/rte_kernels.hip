@@ -288,6 +288,12 @@ __global__ __launch_bounds__(kWave) void k_combine(CombArgs<T, TA> A, const Solv
         for (int l = 0; l <= L; ++l) e += A.coefE[l] * acc[l * l + l];
         S.dE[pix] = e;
     }
+    if (zk == 0 && S.mode != 3) {
+        TA r = TA(0);
+#pragma unroll
+        for (int l = 0; l <= L; ++l) r += A.coefR[l] * acc[l * l + l];
+        S.dR[pix] = r;
+    }
     if (S.mode == 0 || S.mode == 3) {
         const TA ms = TA(S.mus[v]);
         TA gl = TA(1), bs = TA(0);
@@ -310,9 +316,9 @@ __global__ __launch_bounds__(kWave) void k_combine(CombArgs<T, TA> A, const Solv
 // ---------------------------------------------------------------------------
 // Per-solve reduction of the bottom face in a fixed order (deterministic,
 // independent of the batch).  Per-solve mode:
-//   0 (term):  E += dE; red = (sum dE, sum bsum)
+//   0 (term):  E += dE, R += dR; red = (sum dE, sum bsum)
 //   1 (init):  red = (sum E, sum bsum)          (after the source)
-//   2 (final): E += dE; red = (sum dE, 0)
+//   2 (final): E += dE, R += dR; red = (sum dE, 0)
 // grid = nsolve, block = 256.
 // ---------------------------------------------------------------------------
 template <typename T, typename TA>
@@ -327,6 +333,7 @@ __global__ __launch_bounds__(256) void k_bottom(int P, const SolveDev<T, TA> *__
         } else {
             const TA d = S.dE[p];
             S.E[p] += d;
+            S.R[p] += S.dR[p];
             a += d;
         }
         if (S.mode != 2) b += S.bsum[p];
@@ -396,6 +403,8 @@ __global__ __launch_bounds__(kWave) void k_source(GridDev<T> g, double dxd, doub
     S.E[pix] = TA(in ? B.fact * Tc : 0.0);
     S.bsum[pix] = TA(bs);
     S.dE[pix] = TA(0);
+    S.R[pix] = TA(0);
+    S.dR[pix] = TA(0);
 }
 
 // ---------------------------------------------------------------------------

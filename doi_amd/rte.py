@@ -102,6 +102,8 @@ SIGNATURES = {
                             C.POINTER(Params), _dp, C.POINTER(Stats)]),
     "rte_solve_media": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(Beam), C.POINTER(Params),
                                   _dp, C.POINTER(Stats)]),
+    "rte_solve_images": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(Beam), C.POINTER(Params),
+                                   _dp, _dp, C.POINTER(Stats)]),
     "rte_exitance_vjp": (C.c_int, [_vp, _vp, C.POINTER(Beam), C.POINTER(Params), C.POINTER(GradOpts), _dp, _dp,
                                    _dp, _dp, C.POINTER(Stats)]),
     "rte_gradient": (C.c_int, [_vp, _vp, C.POINTER(Beam), C.POINTER(Params), C.POINTER(GradOpts), _dp, _dp, _dp,
@@ -244,6 +246,18 @@ class Context:
         return img.reshape(nb, ny, nx), [dict(terms=s.terms, lam=s.lambda_last, rel=s.rel_change_last,
                                               total=s.total_exitance, ballistic=s.ballistic_total,
                                               gpu_ms=s.gpu_ms, latency_ms=s.latency_ms) for s in st]
+
+    def solve_images(self, media, beams, params):
+        """(transmitted images, reflectance images, stats) of a batch (rte_solve_images)."""
+        nb = len(media)
+        nx, ny, _ = media[0].shape
+        arr = (_vp * nb)(*[m.h for m in media])
+        barr = (Beam * nb)(*[Beam(*b) for b in beams])
+        img = np.zeros(nb * nx * ny)
+        refl = np.zeros(nb * nx * ny)
+        st = (Stats * nb)()
+        _check(lib().rte_solve_images(self.h, nb, arr, barr, C.byref(params), _p(img), _p(refl), st))
+        return img.reshape(nb, ny, nx), refl.reshape(nb, ny, nx), [_stats_dict(s) for s in st]
 
     def solve(self, shape, bounds, mu_a, mu_s, g, beam, params, n=1.0, c=2.9979e10):
         m = self.medium(shape, bounds, mu_a, mu_s, g, n, c)

@@ -7,8 +7,10 @@ The stop rule is the CPU package's lambda rule (CPU package/src/RTE_uniform.c:95
 unless --fixed-terms, which runs nTerms terms as the GPU package's Neumann does
 (GPU package/src/neumann.cu:54), or --tol, which stops at a relative change of
 the total exitance below tol.  The transmitted image (with the ballistic term)
-is written to [Runtime] TransFile in the OutputImage format, imaginary part 0
-(a continuous-wave solve).  ReflectFile and PhaseFile are not produced.
+and the top-face reflectance are written to [Runtime] TransFile and ReflectFile
+in the OutputImage format, imaginary part 0 (a continuous-wave solve); with
+--out only the transmitted image is written, there.  PhaseFile (the
+frequency-domain solve) is not produced.
 """
 import argparse
 import sys
@@ -34,20 +36,25 @@ def solve_file(path, precision="mixed", fixed_terms=False, tol=None, out=None, c
     own = ctx is None
     ctx = ctx or rte.Context(0)
     try:
-        img, st = ctx.solve(prob.shape, prob.bounds, prob.mu_a, prob.mu_s, prob.g, prob.beam, params,
-                            n=prob.n, c=prob.c)
+        med = ctx.medium(prob.shape, prob.bounds, prob.mu_a, prob.mu_s, prob.g, n=prob.n, c=prob.c)
+        try:
+            imgs, refls, sts = ctx.solve_images([med], [prob.beam], params)
+        finally:
+            med.close()
     finally:
         if own:
             ctx.close()
+    img, refl, st = imgs[0], refls[0], sts[0]
     dest = out or prob.trans_file
     if dest:
         refio.write_image(dest, img)
+    if prob.reflect_file and not out:
+        refio.write_image(prob.reflect_file, refl)
     if not quiet:
         print("Neumann series: %d" % st["terms"])
-        for name, what in ((prob.reflect_file, "ReflectFile"), (prob.phase_file, "PhaseFile")):
-            if name:
-                print("note: %s (%s) is not produced" % (what, name), file=sys.stderr)
-    return img, st
+        if prob.phase_file:
+            print("note: PhaseFile (%s) is not produced" % prob.phase_file, file=sys.stderr)
+    return img, st, refl
 
 
 def main(argv=None):

@@ -163,6 +163,13 @@ int rte_solve(rte_ctx *ctx, const rte_grid *grid, const rte_optics *optics,
 int rte_solve_media(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_beam *beams,
                     const rte_params *params, double *exitance, rte_stats *stats);
 
+/* as rte_solve_media, with each solve's top-face reflectance image as well
+ * (ComputeReflImage, CPU package/src/Dist.c:134-171, written to ReflectFile by
+ * CPU package/src/RTE_uniform.c:183-184; no ballistic term); reflectance is
+ * nb*nx*ny and may be NULL */
+int rte_solve_images(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_beam *beams,
+                     const rte_params *params, double *exitance, double *reflectance, rte_stats *stats);
+
 /* ---- derivatives ----------------------------------------------------------
  * Reverse mode through the Neumann series as it ran (same term count): one
  * adjoint series with the transpose of A (reciprocity) and one back-projection

@@ -72,6 +72,10 @@ def lib():
                                 C.POINTER(Params), dp, C.POINTER(Stats)]
         L.orc_solve_field.argtypes = [C.POINTER(Grid), C.POINTER(Optics), C.POINTER(Beam),
                                       C.POINTER(Params), dp, dp, C.POINTER(Stats)]
+        L.orc_solve_images.argtypes = [C.POINTER(Grid), C.POINTER(Optics), C.POINTER(Beam), C.POINTER(Params),
+                                       dp, dp, C.POINTER(Stats)]
+        L.orc_reflectance.argtypes = [C.POINTER(Grid), C.POINTER(Optics), C.c_int, C.c_int, dp, dp]
+        L.orc_quad_weights_refl.argtypes = [C.c_int, C.c_int, dp]
         L.orc_ref_derivative.argtypes = [C.POINTER(Grid), C.POINTER(Optics), C.POINTER(Beam), C.POINTER(Params),
                                          C.POINTER(C.c_int), C.c_int, dp, C.POINTER(Stats)]
         L.orc_bottom_sum.argtypes = [C.POINTER(Grid), C.c_int, dp]
@@ -103,13 +107,18 @@ class Problem:
         self.N = N
 
     def solve(self, beam, L, stop_rule=STOP_LAMBDA, tol=1e-6, max_terms=1000,
-              prop_thresh=1e30, exit_quad=QUAD_REF, field=False):
+              prop_thresh=1e30, exit_quad=QUAD_REF, field=False, reflectance=False):
         nx, ny, _ = self.shape
         img = np.zeros(nx * ny)
         st = Stats()
         prm = Params(L, max_terms, stop_rule, tol, prop_thresh, exit_quad)
         b = Beam(*beam)
-        if field:
+        refl = None
+        if reflectance:
+            refl = np.zeros(nx * ny)
+            rc = lib().orc_solve_images(C.byref(self.grid), C.byref(self.optics), C.byref(b), C.byref(prm),
+                                        _p(img), _p(refl), C.byref(st))
+        elif field:
             w = np.zeros((L + 1) ** 2 * self.N)
             rc = lib().orc_solve_field(C.byref(self.grid), C.byref(self.optics), C.byref(b),
                                        C.byref(prm), _p(img), _p(w), C.byref(st))
@@ -122,6 +131,8 @@ class Problem:
                    rel=st.rel_change_last, ballistic_total=st.ballistic_total)
         if field:
             out["w"] = w.reshape((L + 1) ** 2, self.N)
+        if reflectance:
+            out["reflectance"] = refl.reshape(ny, nx)
         return out
 
     def ref_derivative(self, beam, L, region, which, stop_rule=STOP_LAMBDA, tol=1e-6, max_terms=1000,
@@ -166,6 +177,11 @@ class Problem:
         lib().orc_scatter(C.byref(self.grid), C.byref(self.optics), L, _p(W))
         return W
 
+    def reflectance(self, w, L, quad=QUAD_REF):
+        img = np.zeros(self.shape[0] * self.shape[1])
+        lib().orc_reflectance(C.byref(self.grid), C.byref(self.optics), L, quad, _p(_f64(w)), _p(img))
+        return img
+
     def exitance(self, w, L, quad=QUAD_REF):
         img = np.zeros(self.shape[0] * self.shape[1])
         lib().orc_exitance(C.byref(self.grid), C.byref(self.optics), L, quad, _p(_f64(w)), _p(img))
@@ -177,6 +193,12 @@ class Problem:
 
     def bottom_sum(self, W, L):
         return lib().orc_bottom_sum(C.byref(self.grid), L, _p(_f64(W)))
+
+
+def quad_weights_refl(L, quad=QUAD_REF):
+    R = np.zeros(L + 1)
+    lib().orc_quad_weights_refl(L, quad, _p(R))
+    return R
 
 
 def ylm_complex(L, theta, phi):

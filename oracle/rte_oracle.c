@@ -558,6 +558,48 @@ void orc_quad_weights(int L, int quad, double *Q)
     }
 }
 
+/* R_l = int_{-1}^{0} P_l(u) u du: the reference's left Riemann sum over
+ * u = p/100 - 1, p = 0..99 (CPU package/src/Dist.c:146-160), or exactly,
+ * R_l = (-1)^(l+1) int_0^1 P_l(u) u du since P_l(-u) = (-1)^l P_l(u). */
+void orc_quad_weights_refl(int L, int quad, double *R)
+{
+    if (quad == ORC_QUAD_REF) {
+        for (int l = 0; l <= L; ++l) {
+            double s = 0.0;
+            for (int p = 0; p < 100; ++p) {
+                double u = (double)p / 100.0 - 1.0;
+                s += orc_assoc_legendre(u, l, 0) * u / 100.0;
+            }
+            R[l] = s;
+        }
+        return;
+    }
+    orc_quad_weights(L, quad, R);
+    for (int l = 0; l <= L; ++l) R[l] *= (l & 1) ? 1.0 : -1.0;
+}
+
+/* Top-face reflectance R_ij = -sum_l 2 pi dx dy c_m / sqrt(4 pi) sqrt(2l+1) R_l
+ * w_l0(i, j, 0) (CPU package/src/Dist.c:134-171); no ballistic term. */
+void orc_reflectance(const orc_grid *g, const orc_optics *o, int L, int quad, const double *w,
+                     double *img)
+{
+    geo G;
+    geo_init(&G, g);
+    double R[128];
+    orc_quad_weights_refl(L, quad, R);
+    double c_m = o->c / o->n;
+    for (int j = 0; j < G.ny; ++j)
+        for (int i = 0; i < G.nx; ++i) {
+            int v = i + G.nx * j;
+            double val = 0.0;
+            for (int l = 0; l <= L; ++l)
+                val -= 2.0 * M_PI * G.dx * G.dy * c_m / sqrt(4.0 * M_PI) * sqrt(2.0 * l + 1.0) * R[l] *
+                       w[(size_t)(l * l + l) * G.N + v];
+            img[i + G.nx * j] = val;
+        }
+    geo_free(&G);
+}
+
 /* E_ij = sum_l 2 pi dx dy c_m / sqrt(4 pi) sqrt(2l+1) Q_l w_l0(i, j, nz-1)
  * (CPU package/src/Dist.c:211-218), ballistic term excluded. */
 void orc_exitance(const orc_grid *g, const orc_optics *o, int L, int quad, const double *w,
@@ -619,7 +661,7 @@ static int stop_lambda(double *fi, int n, double sumW, double sumOut, double *la
  * bottom exitance of A sum_q (DA)^q s0 (ball may be NULL: no ballistic term);
  * w_out (may be NULL) receives A sum_q (DA)^q s0. */
 static int neumann_from(const orc_grid *g, const orc_optics *o, const orc_params *p, const double *s0,
-                        const double *ball, double *exitance, double *w_out, orc_stats *st)
+                        const double *ball, double *exitance, double *w_out, double *refl, orc_stats *st)
 {
     int L = p->L, K = ncoef(L), N = g->nx * g->ny * g->nz, P = g->nx * g->ny;
     double *W = malloc(sizeof(double) * (size_t)K * N);
@@ -652,6 +694,7 @@ static int neumann_from(const orc_grid *g, const orc_optics *o, const orc_params
         orc_propagate(g, o, L, p->prop_thresh, out, AW);
         orc_exitance(g, o, L, p->exit_quad, AW, img);
         for (int i = 0; i < P; ++i) exitance[i] = (ball ? ball[i] : 0.0) + img[i];
+        if (refl) orc_reflectance(g, o, L, p->exit_quad, AW, refl);
         if (w_out) memcpy(w_out, AW, sizeof(double) * (size_t)K * N);
     } else {
         /* relative change of the total exitance: by linearity of A the exitance
@@ -659,6 +702,7 @@ static int neumann_from(const orc_grid *g, const orc_optics *o, const orc_params
         double T = btot;
         for (int i = 0; i < P; ++i) exitance[i] = ball ? ball[i] : 0.0;
         if (w_out) memset(w_out, 0, sizeof(double) * (size_t)K * N);
+        if (refl) memset(refl, 0, sizeof(double) * P);
         n = 0;
         for (;;) {
             orc_propagate(g, o, L, p->prop_thresh, W, AW);
@@ -666,6 +710,11 @@ static int neumann_from(const orc_grid *g, const orc_optics *o, const orc_params
             double dT = 0.0;
             for (int i = 0; i < P; ++i) { exitance[i] += img[i]; dT += img[i]; }
             if (w_out) for (size_t i = 0; i < (size_t)K * N; ++i) w_out[i] += AW[i];
+            if (refl) {
+                orc_reflectance(g, o, L, p->exit_quad, AW, img);
+                for (int i = 0; i < P; ++i) refl[i] += img[i];
+                orc_exitance(g, o, L, p->exit_quad, AW, img);
+            }
             T += dT;
             ++n;
             rel = (T != 0.0) ? fabs(dT) / fabs(T) : 0.0;
@@ -685,13 +734,13 @@ static int neumann_from(const orc_grid *g, const orc_optics *o, const orc_params
 }
 
 static int solve_impl(const orc_grid *g, const orc_optics *o, const orc_beam *b, const orc_params *p,
-                      double *exitance, double *w_out, orc_stats *st)
+                      double *exitance, double *w_out, double *refl, orc_stats *st)
 {
     int L = p->L, K = ncoef(L), N = g->nx * g->ny * g->nz, P = g->nx * g->ny;
     double *s0 = malloc(sizeof(double) * (size_t)K * N);
     double *ball = malloc(sizeof(double) * P);
     int rc = -1;
-    if (orc_source(g, o, b, L, s0, ball)) rc = neumann_from(g, o, p, s0, ball, exitance, w_out, st);
+    if (orc_source(g, o, b, L, s0, ball)) rc = neumann_from(g, o, p, s0, ball, exitance, w_out, refl, st);
     free(s0);
     free(ball);
     return rc;
@@ -700,7 +749,13 @@ static int solve_impl(const orc_grid *g, const orc_optics *o, const orc_beam *b,
 int orc_solve(const orc_grid *g, const orc_optics *o, const orc_beam *b, const orc_params *p,
               double *exitance, orc_stats *st)
 {
-    return solve_impl(g, o, b, p, exitance, NULL, st);
+    return solve_impl(g, o, b, p, exitance, NULL, NULL, st);
+}
+
+int orc_solve_images(const orc_grid *g, const orc_optics *o, const orc_beam *b, const orc_params *p,
+                     double *exitance, double *reflectance, orc_stats *st)
+{
+    return solve_impl(g, o, b, p, exitance, NULL, reflectance, st);
 }
 
 /* FisherInfo_IO/src/compute_grad_rte.cu:241-301 (mu_a) and :341-371 (mu_s):
@@ -720,14 +775,14 @@ int orc_ref_derivative(const orc_grid *g, const orc_optics *o, const orc_beam *b
     int L = p->L, K = ncoef(L), N = g->nx * g->ny * g->nz, P = g->nx * g->ny;
     double *phi = malloc(sizeof(double) * (size_t)K * N);
     double *img = malloc(sizeof(double) * P);
-    int rc = solve_impl(g, o, b, p, img, phi, NULL);
+    int rc = solve_impl(g, o, b, p, img, phi, NULL, NULL);
     if (rc == 0) {
         for (int a = 0; a < K; ++a) {
             const int l = (int)sqrt((double)a + 0.5);
             const double f = (o->c / o->n) * (which == 0 ? -1.0 : pow(o->g, (double)l) - 1.0);
             for (int v = 0; v < N; ++v) phi[(size_t)a * N + v] = region[v] ? f * phi[(size_t)a * N + v] : 0.0;
         }
-        rc = neumann_from(g, o, p, phi, NULL, dimg, NULL, st);
+        rc = neumann_from(g, o, p, phi, NULL, dimg, NULL, NULL, st);
     }
     free(phi);
     free(img);
@@ -737,5 +792,5 @@ int orc_ref_derivative(const orc_grid *g, const orc_optics *o, const orc_beam *b
 int orc_solve_field(const orc_grid *g, const orc_optics *o, const orc_beam *b, const orc_params *p,
                     double *exitance, double *w_out, orc_stats *st)
 {
-    return solve_impl(g, o, b, p, exitance, w_out, st);
+    return solve_impl(g, o, b, p, exitance, w_out, NULL, st);
 }

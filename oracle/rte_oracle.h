@@ -92,10 +92,18 @@ void orc_scatter(const orc_grid *g, const orc_optics *o, int L, double *W);
 void orc_exitance(const orc_grid *g, const orc_optics *o, int L, int quad,
                   const double *w, double *img);
 void orc_quad_weights(int L, int quad, double *Q);
+/* top-face reflectance of a field (nx*ny) and its quadrature int_{-1}^0 P_l(u) u du */
+void orc_reflectance(const orc_grid *g, const orc_optics *o, int L, int quad,
+                     const double *w, double *img);
+void orc_quad_weights_refl(int L, int quad, double *R);
 
 /* full solve: source -> Neumann -> bottom exitance (nx*ny) incl. ballistic */
 int orc_solve(const orc_grid *g, const orc_optics *o, const orc_beam *b,
               const orc_params *p, double *exitance, orc_stats *st);
+
+/* full solve with the top-face reflectance image as well (nx*ny each) */
+int orc_solve_images(const orc_grid *g, const orc_optics *o, const orc_beam *b,
+                     const orc_params *p, double *exitance, double *reflectance, orc_stats *st);
 
 /* full solve keeping the final radiance w = A out (K*N) for gradient checks */
 int orc_solve_field(const orc_grid *g, const orc_optics *o, const orc_beam *b,

@@ -4,7 +4,8 @@ its own sources by oracle/Makefile.ref into oracle/_ref/rte_cpu.
 
 Run in the build container only (it needs /root/reference to build the
 binary):  make -f oracle/Makefile.ref && python tests/golden/gen_ref_fixtures.py
-The fixture holds data only: the .par inputs and the binary's outputs.
+The fixture holds data only: the .par inputs and the binary's outputs
+(transmitted and reflected images, term count, printed photon total).
 """
 import json
 import os
@@ -80,20 +81,24 @@ def run_case(case):
         out = subprocess.run([BIN, "in.ini"], cwd=d, capture_output=True, text=True, check=True,
                              env=dict(os.environ, OMP_NUM_THREADS="8")).stdout
         terms = int(re.findall(r"Neumann series: (\d+)", out)[-1])
-        raw = open(os.path.join(d, "trans.out"), "rb").read()
-        rx, ry = struct.unpack("ii", raw[:8])
-        vals = np.frombuffer(raw[8:], dtype=np.float64)
-        real = vals[: rx * ry].reshape(ry, rx)
+        photons = float(re.findall(r"Number of photons total: ([-+0-9.eE]+)", out)[-1])
+        images = {}
+        for key, fname in (("trans_image", "trans.out"), ("refl_image", "refl.out")):
+            raw = open(os.path.join(d, fname), "rb").read()
+            rx, ry = struct.unpack("ii", raw[:8])
+            vals = np.frombuffer(raw[8:], dtype=np.float64)
+            images[key] = vals[: rx * ry].reshape(ry, rx).tolist()
     return dict(name=name, shape=[nx, ny, nz], bounds=list(b), mua=mua, mus=mus, g=g, n=n, L=L,
                 beam=dict(pos_x=beam[0], pos_y=beam[1], diam=beam[2], mag=1.0), prop_thresh=pt,
-                terms=terms, trans_image=real.tolist())
+                terms=terms, photons_printed=photons, **images)
 
 
 def main():
     res = [run_case(c) for c in CASES]
     with open(os.path.join(HERE, "ref_cpu_uniform.json"), "w") as f:
         json.dump(dict(source="reference CPU package (CPU package/src), built by oracle/Makefile.ref",
-                       image_layout="trans_image[j][i], j = y index, i = x index (OutputImage order)",
+                       image_layout="trans_image / refl_image [j][i], j = y index, i = x index (OutputImage "
+                                    "order); photons_printed: the binary's 'Number of photons total' (%f)",
                        cases=res), f, indent=1)
     for r in res:
         print(r["name"], r["terms"], np.sum(r["trans_image"]))

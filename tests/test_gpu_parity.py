@@ -103,17 +103,18 @@ def test_path_integrals(ctx):
 @pytest.mark.parametrize("prec,tol", [(rte.PREC_FP64, 1e-9), (rte.PREC_MIXED, 1e-5), (rte.PREC_FP32, 1e-4)])
 def test_golden_reference_cases(ctx, golden_ref, prec, tol):
     """Every uniform case the reference binary produced: same term count, and
-    the exitance within the precision's tolerance."""
+    the transmitted and reflected images within the precision's tolerance."""
     for c in golden_ref:
         b = c["beam"]
         p = rte.make_params(c["L"], precision=prec, stop_rule=rte.STOP_LAMBDA,
                             prop_thresh=c["prop_thresh"])
         print("golden", c["name"], prec, flush=True)
-        img, st = ctx.solve(tuple(c["shape"]), tuple(c["bounds"]), c["mua"], c["mus"], c["g"],
-                            (b["pos_x"], b["pos_y"], b["diam"], b["mag"]), p, n=c["n"])
-        ref = np.array(c["trans_image"])
-        assert st["terms"] == c["terms"], c["name"]
-        assert rel_max(img, ref) < tol, c["name"]
+        m = ctx.medium(tuple(c["shape"]), tuple(c["bounds"]), c["mua"], c["mus"], c["g"], n=c["n"])
+        imgs, refls, sts = ctx.solve_images([m], [(b["pos_x"], b["pos_y"], b["diam"], b["mag"])], p)
+        m.close()
+        assert sts[0]["terms"] == c["terms"], c["name"]
+        assert rel_max(imgs[0], np.array(c["trans_image"])) < tol, c["name"]
+        assert rel_max(refls[0], np.array(c["refl_image"])) < tol, c["name"]
 
 
 @pytest.mark.parametrize("rule", [rte.STOP_LAMBDA, rte.STOP_DELTA])

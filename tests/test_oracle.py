@@ -22,12 +22,39 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # the reference binary's outputs (uniform media)
 # ---------------------------------------------------------------------------
 def test_reference_golden_cases(golden_ref):
+    """Transmitted and reflected images and term counts of every case the
+    reference binary produced."""
     for c in golden_ref:
         P = O.Problem(tuple(c["shape"]), tuple(c["bounds"]), c["mua"], c["mus"], c["g"], c["n"])
         b = c["beam"]
-        r = P.solve((b["pos_x"], b["pos_y"], b["diam"], b["mag"]), c["L"], prop_thresh=c["prop_thresh"])
+        r = P.solve((b["pos_x"], b["pos_y"], b["diam"], b["mag"]), c["L"], prop_thresh=c["prop_thresh"],
+                    reflectance=True)
         assert r["terms"] == c["terms"], c["name"]
         assert rel_max(r["exitance"], np.array(c["trans_image"])) < 1e-12, c["name"]
+        assert rel_max(r["reflectance"], np.array(c["refl_image"])) < 1e-12, c["name"]
+
+
+def test_reflectance_quadrature():
+    """int_{-1}^0 P_l(u) u du = -1/2, 1/3, -1/8, 0, 1/48, 0 (exact mode), and the
+    reference's 100-point left Riemann sum over u = p/100 - 1 in closed form for
+    l = 0, 1: sum u / 100 = -0.505, sum u^2 / 100 = 0.33835."""
+    R = O.quad_weights_refl(5, O.QUAD_EXACT)
+    assert np.allclose(R, [-0.5, 1 / 3, -0.125, 0.0, 1 / 48, 0.0], atol=1e-14)
+    Rr = O.quad_weights_refl(5, O.QUAD_REF)
+    assert abs(Rr[0] + 0.505) < 1e-15 and abs(Rr[1] - 0.33835) < 1e-15
+    # reflectance of a top-layer field is the exitance functional on the mirrored grid
+    shape = (3, 2, 4)
+    bounds = bounds_for(shape)
+    P = O.Problem(shape, bounds, 0.01, 1.0, 0.5)
+    L, K, N = 3, 16, 24
+    w = np.random.default_rng(2).standard_normal((K, N))
+    flip = w.reshape(K, 4, 6)[:, ::-1, :].reshape(K, N) * np.array(
+        [(-1.0) ** int(np.sqrt(a + 0.5)) for a in range(K)])[:, None]   # z -> -z: R_l0 picks up (-1)^l
+    for q in (O.QUAD_REF, O.QUAD_EXACT):
+        refl = P.reflectance(w, L, q)
+        ex = P.exitance(flip, L, q)
+        if q == O.QUAD_EXACT:
+            assert rel_max(refl, ex) < 1e-13
 
 
 # ---------------------------------------------------------------------------
