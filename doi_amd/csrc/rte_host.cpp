@@ -758,7 +758,7 @@ struct RefSrc {
 template <typename T, typename TA>
 static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_beam *beams,
                      const rte_params *p, double *exitance, rte_stats *stats, const RefSrc<TA> *rs = nullptr,
-                     double *reflectance = nullptr)
+                     double *reflectance = nullptr, double *field = nullptr)
 {
     const rte_grid &grid = media[0]->grid;
     const int L = p->L, K = (L + 1) * (L + 1);
@@ -836,6 +836,13 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
         if (cols == 0) return fail(RTE_E_ARG, "solve %d: beam hits no voxel column", i);
         bd[i] = BeamDev{beams[i].pos_x, beams[i].pos_y, half, beams[i].mag / (double)cols,
                         m->mua_d.as<double>(), m->mus_d.as<double>()};
+    }
+    // the final field sum_q A W_q of solve 0, if asked for
+    DevBuf dphi;
+    if (field) {
+        HIP_TRY(dphi.alloc(szW * sizeof(TA)));
+        HIP_TRY(launch_fill<TA>(dphi.as<TA>(), szW, TA(0), st));
+        sv[0].phi = dphi.as<TA>();
     }
     SolveDev<T, TA> *svd = ctx->ws_sv.as<SolveDev<T, TA>>();
     HIP_TRY(hipMemcpyAsync(svd, sv.data(), sizeof(SolveDev<T, TA>) * nb, hipMemcpyHostToDevice, st));
@@ -988,6 +995,15 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
             for (int q = 0; q < P; ++q) reflectance[(size_t)i * P + q] = (double)img[q];
         }
     }
+    if (field) {
+        // coefficient-major, in the reference's units (A carries 1/c_m there)
+        std::vector<TA> f(szW);
+        HIP_TRY(hipMemcpyAsync(f.data(), dphi.p, sizeof(TA) * szW, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        const double inv_cm = media[0]->n / media[0]->c;
+        for (int v = 0; v < N; ++v)
+            for (int a = 0; a < K; ++a) field[(size_t)a * N + v] = (double)f[(size_t)v * K + a] * inv_cm;
+    }
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     if (stats) {
@@ -1059,6 +1075,27 @@ extern "C" int rte_solve_media(rte_ctx *ctx, int nb, rte_medium *const *media, c
                                const rte_params *params, double *exitance, rte_stats *stats)
 {
     return rte_solve_images(ctx, nb, media, beams, params, exitance, nullptr, stats);
+}
+
+extern "C" int rte_solve_field(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte_params *params,
+                               double *field, double *exitance, double *reflectance, rte_stats *stats)
+{
+    if (!ctx || !m || !beam || !field || !exitance) return fail(RTE_E_ARG, "rte_solve_field: NULL argument");
+    if (m->ctx != ctx) return fail(RTE_E_ARG, "rte_solve_field: medium belongs to another context");
+    int rc;
+    if ((rc = check_params(params))) return rc;
+    if (!std::isfinite(beam->pos_x) || !std::isfinite(beam->pos_y) || !(beam->diam >= 0.0))
+        return fail(RTE_E_ARG, "rte_solve_field: invalid beam");
+    if ((rc = set_device(ctx))) return rc;
+    rte_medium *const media[1] = {m};
+    switch (params->precision) {
+    case RTE_PREC_FP64:
+        return run_batch<double, double>(ctx, 1, media, beam, params, exitance, stats, nullptr, reflectance, field);
+    case RTE_PREC_MIXED:
+        return run_batch<float, double>(ctx, 1, media, beam, params, exitance, stats, nullptr, reflectance, field);
+    default:
+        return run_batch<float, float>(ctx, 1, media, beam, params, exitance, stats, nullptr, reflectance, field);
+    }
 }
 
 extern "C" int rte_solve(rte_ctx *ctx, const rte_grid *grid, const rte_optics *optics,

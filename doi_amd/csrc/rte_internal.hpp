@@ -55,6 +55,7 @@ struct SolveDev {
     TA *R;             // [nx*ny] accumulated reflectance
     TA *red;           // [2] reduced (sum dE, sum bsum) of this term
     TA *aw_store;      // [N][K] if set, k_combine also writes A W (unscaled) here
+    TA *phi;           // [N][K] if set, k_combine adds A W (unscaled) here: the final field
     int mode;          // 0 term, 1 init, 2 final, 3 recompute (see k_bottom / k_combine)
 };
 

@@ -282,6 +282,10 @@ __global__ __launch_bounds__(kWave) void k_combine(CombArgs<T, TA> A, const Solv
 #pragma unroll
         for (int a = 0; a < K; ++a) S.aw_store[(size_t)v * K + a] = acc[a];
     }
+    if (S.phi && S.mode != 3) {
+#pragma unroll
+        for (int a = 0; a < K; ++a) S.phi[(size_t)v * K + a] += acc[a];
+    }
     if (bottom && S.mode != 3) {
         TA e = TA(0);
 #pragma unroll

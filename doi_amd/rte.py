@@ -104,6 +104,7 @@ SIGNATURES = {
                                   _dp, C.POINTER(Stats)]),
     "rte_solve_images": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(Beam), C.POINTER(Params),
                                    _dp, _dp, C.POINTER(Stats)]),
+    "rte_solve_field": (C.c_int, [_vp, _vp, C.POINTER(Beam), C.POINTER(Params), _dp, _dp, _dp, C.POINTER(Stats)]),
     "rte_exitance_vjp": (C.c_int, [_vp, _vp, C.POINTER(Beam), C.POINTER(Params), C.POINTER(GradOpts), _dp, _dp,
                                    _dp, _dp, C.POINTER(Stats)]),
     "rte_gradient": (C.c_int, [_vp, _vp, C.POINTER(Beam), C.POINTER(Params), C.POINTER(GradOpts), _dp, _dp, _dp,
@@ -258,6 +259,18 @@ class Context:
         st = (Stats * nb)()
         _check(lib().rte_solve_images(self.h, nb, arr, barr, C.byref(params), _p(img), _p(refl), st))
         return img.reshape(nb, ny, nx), refl.reshape(nb, ny, nx), [_stats_dict(s) for s in st]
+
+    def solve_field(self, medium, beam, params):
+        """(final field [K, N] in the reference's units, transmitted image, reflectance, stats)."""
+        nx, ny, _ = medium.shape
+        K = (params.L + 1) ** 2
+        field = np.zeros(K * medium.N)
+        img = np.zeros(nx * ny)
+        refl = np.zeros(nx * ny)
+        st = Stats()
+        _check(lib().rte_solve_field(self.h, medium.h, C.byref(Beam(*beam)), C.byref(params), _p(field), _p(img),
+                                     _p(refl), C.byref(st)))
+        return field.reshape(K, medium.N), img.reshape(ny, nx), refl.reshape(ny, nx), _stats_dict(st)
 
     def solve(self, shape, bounds, mu_a, mu_s, g, beam, params, n=1.0, c=2.9979e10):
         m = self.medium(shape, bounds, mu_a, mu_s, g, n, c)

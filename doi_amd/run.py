@@ -10,12 +10,14 @@ the total exitance below tol.  The transmitted image (with the ballistic term)
 and the top-face reflectance are written to [Runtime] TransFile and ReflectFile
 in the OutputImage format, imaginary part 0 (a continuous-wave solve); with
 --out only the transmitted image is written, there.  PhaseFile (the
-frequency-domain solve) is not produced.
+frequency-domain solve) is not produced.  The photon total (outward and inward
+flux through the six faces plus the ballistic transmission,
+CPU package/src/Dist.c:538-988) is printed as the reference prints it.
 """
 import argparse
 import sys
 
-from . import refio, rte
+from . import faces, refio, rte
 
 PRECISIONS = {"fp64": rte.PREC_FP64, "mixed": rte.PREC_MIXED, "fp32": rte.PREC_FP32}
 
@@ -38,19 +40,20 @@ def solve_file(path, precision="mixed", fixed_terms=False, tol=None, out=None, c
     try:
         med = ctx.medium(prob.shape, prob.bounds, prob.mu_a, prob.mu_s, prob.g, n=prob.n, c=prob.c)
         try:
-            imgs, refls, sts = ctx.solve_images([med], [prob.beam], params)
+            field, img, refl, st = ctx.solve_field(med, prob.beam, params)
         finally:
             med.close()
     finally:
         if own:
             ctx.close()
-    img, refl, st = imgs[0], refls[0], sts[0]
+    st["photons"] = faces.photon_number(field, prob.shape, prob.bounds, prob.L, st["ballistic"], c=prob.c, n=prob.n)
     dest = out or prob.trans_file
     if dest:
         refio.write_image(dest, img)
     if prob.reflect_file and not out:
         refio.write_image(prob.reflect_file, refl)
     if not quiet:
+        print("Number of photons total: %f" % st["photons"])
         print("Neumann series: %d" % st["terms"])
         if prob.phase_file:
             print("note: PhaseFile (%s) is not produced" % prob.phase_file, file=sys.stderr)

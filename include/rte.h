@@ -170,6 +170,15 @@ int rte_solve_media(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_be
 int rte_solve_images(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_beam *beams,
                      const rte_params *params, double *exitance, double *reflectance, rte_stats *stats);
 
+/* one solve that also returns the final radiance field, the reference's
+ * out2 = A (s0 + sum W) (CPU package/src/RTE_uniform.c:95-115, OutputDist's DistFile),
+ * coefficient-major field[a*N + v] in the real SH basis and the reference's units
+ * (A including 1/c_m); the boundary-face images and the photon count of the
+ * reference (CPU package/src/Dist.c:349-989) are quadratures of it.
+ * reflectance (nx*ny) may be NULL. */
+int rte_solve_field(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte_params *params,
+                    double *field, double *exitance, double *reflectance, rte_stats *stats);
+
 /* ---- derivatives ----------------------------------------------------------
  * Reverse mode through the Neumann series as it ran (same term count): one
  * adjoint series with the transpose of A (reciprocity) and one back-projection

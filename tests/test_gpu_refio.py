@@ -6,8 +6,9 @@ term count writes both files."""
 import numpy as np
 import pytest
 
-from doi_amd import refio, run
-from tests.helpers import rel_max
+from doi_amd import faces, refio, rte, run
+from oracle import oracle as O
+from tests.helpers import bounds_for, hetero_maps, rel_max
 from tests.test_refio import ini_for
 
 pytestmark = pytest.mark.gpu
@@ -24,6 +25,7 @@ def test_parameter_file_end_to_end(ctx, golden_ref, tmp_path):
         assert real.shape == (case["shape"][1], case["shape"][0]) and not np.any(imag)
         assert rel_max(real, np.array(case["trans_image"])) < 1e-9, case["name"]
         assert rel_max(refl, np.array(case["refl_image"])) < 1e-9, case["name"]
+        assert abs(st["photons"] - case["photons_printed"]) <= 5.0e-7 + 1e-9, case["name"]
     # the command line: nTerms = 30 terms, mixed precision, against the lambda-rule FP64 image
     case = golden_ref[0]
     p = tmp_path / "cli.ini"
@@ -44,3 +46,26 @@ def test_parameter_file_end_to_end(ctx, golden_ref, tmp_path):
         os.chdir(cwd)
     assert rel_max(t, np.array(case["trans_image"])) < 1e-4
     assert rel_max(r, np.array(case["refl_image"])) < 1e-4
+
+
+def test_final_field_against_oracle(ctx):
+    """rte_solve_field: the final radiance field (reference units) of a
+    heterogeneous solve against the oracle's (FP64 1e-10, mixed 1e-5 of the
+    largest coefficient), and the face fluxes computed from it."""
+    shape, L = (6, 5, 4), 3
+    bounds = bounds_for(shape)
+    ma, ms = hetero_maps(shape, seed=71)
+    beam = (0.05, 0.0, 0.1, 1.0)
+    ref = O.Problem(shape, bounds, ma, ms, 0.8, n=1.2).solve(beam, L, stop_rule=O.STOP_DELTA, tol=1e-7,
+                                                           max_terms=30, field=True)
+    m = ctx.medium(shape, bounds, ma, ms, 0.8, n=1.2)
+    for prec, tol in ((rte.PREC_FP64, 1e-10), (rte.PREC_MIXED, 1e-5)):
+        p = rte.make_params(L, precision=prec, stop_rule=rte.STOP_DELTA, tol=1e-7, max_terms=30)
+        field, img, refl, st = ctx.solve_field(m, beam, p)
+        assert st["terms"] == ref["terms"]
+        assert np.max(np.abs(field - ref["w"])) < tol * np.max(np.abs(ref["w"]))
+        assert rel_max(img, ref["exitance"]) < tol
+        t_gpu = faces.photon_number(field, shape, bounds, L, st["ballistic"], n=1.2)
+        t_orc = faces.photon_number(ref["w"], shape, bounds, L, ref["ballistic_total"], n=1.2)
+        assert abs(t_gpu - t_orc) < 10 * tol * abs(t_orc)
+    m.close()

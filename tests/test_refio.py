@@ -150,3 +150,23 @@ def test_parameter_files_reproduce_reference_images(tmp_path, golden_ref):
         r = P.solve(prob.beam, prob.L, stop_rule=O.STOP_LAMBDA, max_terms=1000, prop_thresh=prob.prop_thresh)
         assert r["terms"] == case["terms"], case["name"]
         assert rel_max(r["exitance"], np.array(case["trans_image"])) < 1e-12, case["name"]
+
+
+def test_face_quadratures_reproduce_reference_photon_totals(golden_ref):
+    """doi_amd.faces on the oracle's final field reproduces the photon total
+    the reference binary printed (%f, 6 decimals) for every golden case; its
+    real harmonics equal the oracle's."""
+    from doi_amd import faces
+    rng = np.random.default_rng(1)
+    th, ph = rng.uniform(0, np.pi, 6), rng.uniform(0, 2 * np.pi, 6)
+    R = faces.real_sh(6, th, ph)
+    for k in range(6):
+        assert np.max(np.abs(R[:, k] - O.ylm_real(6, th[k], ph[k]))) < 1e-13
+    for c in golden_ref:
+        b = c["beam"]
+        P = O.Problem(tuple(c["shape"]), tuple(c["bounds"]), c["mua"], c["mus"], c["g"], n=c["n"])
+        r = P.solve((b["pos_x"], b["pos_y"], b["diam"], b["mag"]), c["L"], prop_thresh=c["prop_thresh"],
+                    field=True)
+        tot = faces.photon_number(r["w"], tuple(c["shape"]), tuple(c["bounds"]), c["L"], r["ballistic_total"],
+                                  n=c["n"])
+        assert abs(tot - c["photons_printed"]) <= 5.0e-7 + 1e-12, c["name"]
