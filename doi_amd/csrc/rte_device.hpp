@@ -26,6 +26,13 @@ template <>
 __device__ __forceinline__ double dev_exp<double>(double x) { return exp(x); }
 
 template <typename T>
+__device__ __forceinline__ void dev_sincos(T x, T *s, T *c);
+template <>
+__device__ __forceinline__ void dev_sincos<float>(float x, float *s, float *c) { sincosf(x, s, c); }
+template <>
+__device__ __forceinline__ void dev_sincos<double>(double x, double *s, double *c) { sincos(x, s, c); }
+
+template <typename T>
 __device__ __forceinline__ T big_value();
 template <>
 __device__ __forceinline__ float big_value<float>() { return 1.0e30f; }

@@ -684,6 +684,7 @@ struct HostSolve {
     int state;  // 0 term, 2 final, 3 done
     int n;
     double T, sumOut, fi, lam, rel, ball;
+    double Ti = 0.0;  // imaginary part of the total (frequency-domain solves)
 };
 
 static Scales make_scales(int L)
@@ -711,18 +712,34 @@ static double stop_weight(int a)
     return 0.0;
 }
 
+// The same sums for the imaginary half of a complex field: with
+// c_lm = (r_lm - i r_l,-m)/sqrt2 and c_l,-m = (-1)^m (r_lm + i r_l,-m)/sqrt2,
+// Re c_lm + Re c_l,-m = sqrt2 Im r_l,-m for odd m (CPU package/src/
+// Kernels_selfsub.c:1194-1206 takes the real part of every complex coefficient).
+static double stop_weight_imag(int a)
+{
+    int l = 0;
+    while ((l + 1) * (l + 1) <= a) ++l;
+    const int m = a - l * l - l;
+    return (m < 0 && ((-m) % 2) == 1) ? std::sqrt(2.0) : 0.0;
+}
+
 // One term of the Neumann series has been propagated: fold its exitance
 // increment and stop-rule sum into the series state and decide what comes
 // next (0 another term, 2 the final exitance-only propagation, 3 done).
-static void advance_series(HostSolve &h, double dT, double sumW, int rule, double tol, int max_terms)
+static void advance_series(HostSolve &h, double dT, double sumW, int rule, double tol, int max_terms,
+                           double dTi = 0.0)
 {
     h.T += dT;
+    h.Ti += dTi;
     if (h.state == 2) {
         h.state = 3;
         return;
     }
     h.n += 1;
-    h.rel = h.T != 0.0 ? std::fabs(dT) / std::fabs(h.T) : 0.0;
+    // |dT| / |T| of the complex total (Ti = dTi = 0 for a continuous-wave solve)
+    const double aT = std::hypot(h.T, h.Ti);
+    h.rel = aT != 0.0 ? std::hypot(dT, dTi) / aT : 0.0;
     if (rule == RTE_STOP_DELTA) {
         if (h.rel < tol || h.n >= max_terms) h.state = 3;
         return;
@@ -758,8 +775,12 @@ struct RefSrc {
 template <typename T, typename TA>
 static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_beam *beams,
                      const rte_params *p, double *exitance, rte_stats *stats, const RefSrc<TA> *rs = nullptr,
-                     double *reflectance = nullptr, double *field = nullptr)
+                     double *reflectance = nullptr, double *field = nullptr, const double *kim = nullptr)
 {
+    // frequency-domain solves (kim != nullptr) run as two real halves each,
+    // solve i = (sv[2i], sv[2i+1]) = (real, imaginary) part of its complex field
+    const int hv = kim ? 2 : 1, ns = nb * hv;
+    if (kim && rs) return fail(RTE_E_UNSUPP, "derivative solves are continuous-wave only");
     const rte_grid &grid = media[0]->grid;
     const int L = p->L, K = (L + 1) * (L + 1);
     const GridDev<T> g = make_grid<T>(grid);
@@ -798,28 +819,32 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
     const int clen = (N + nchunk - 1) / nchunk;
     const int nch = (N + clen - 1) / clen;
     const size_t szW = (size_t)N * K, szPart = (size_t)nch * N * K;
-    HIP_TRY(ctx->ws_W.alloc(szW * sizeof(T) * nb));
-    HIP_TRY(ctx->ws_part.alloc(szPart * sizeof(T) * nb));
-    HIP_TRY(ctx->ws_pix.alloc((size_t)kPixBufs * P * sizeof(TA) * nb));
-    HIP_TRY(ctx->ws_red.alloc((size_t)2 * sizeof(TA) * nb));
-    HIP_TRY(ctx->ws_sv.alloc(sizeof(SolveDev<T, TA>) * nb * 2));
+    HIP_TRY(ctx->ws_W.alloc(szW * sizeof(T) * ns));
+    HIP_TRY(ctx->ws_part.alloc(szPart * sizeof(T) * ns));
+    HIP_TRY(ctx->ws_pix.alloc((size_t)kPixBufs * P * sizeof(TA) * ns));
+    HIP_TRY(ctx->ws_red.alloc((size_t)2 * sizeof(TA) * ns));
+    HIP_TRY(ctx->ws_sv.alloc(sizeof(SolveDev<T, TA>) * ns * 2));
     HIP_TRY(ctx->ws_beams.alloc(sizeof(BeamDev) * nb));
 
-    std::vector<SolveDev<T, TA>> sv(nb);
+    std::vector<SolveDev<T, TA>> sv(ns);
     std::vector<BeamDev> bd(nb);
     std::vector<HostSolve> hs(nb);
     for (int i = 0; i < nb; ++i) {
         rte_medium *m = media[i];
-        SolveDev<T, TA> &s = sv[i];
-        s.W = ctx->ws_W.as<T>() + szW * i;
-        s.part = ctx->ws_part.as<T>() + szPart * i;
-        s.ktab = sizeof(T) == 4 ? (const T *)m->ktab_f.p : (const T *)m->ktab_d.p;
-        s.mut = sizeof(T) == 4 ? (const T *)m->mut_f.p : (const T *)m->mut_d.p;
-        s.mus = sizeof(T) == 4 ? (const T *)m->mus_f.p : (const T *)m->mus_d.p;
-        s.g = T(m->g);
-        set_pixels(s, ctx->ws_pix.as<TA>() + (size_t)kPixBufs * P * i, P);
-        s.red = ctx->ws_red.as<TA>() + 2 * i;
-        s.mode = 1;
+        for (int hh = 0; hh < hv; ++hh) {
+            const int q = i * hv + hh;
+            SolveDev<T, TA> &s = sv[q];
+            s.W = ctx->ws_W.as<T>() + szW * q;
+            s.part = ctx->ws_part.as<T>() + szPart * q;
+            s.ktab = sizeof(T) == 4 ? (const T *)m->ktab_f.p : (const T *)m->ktab_d.p;
+            s.mut = sizeof(T) == 4 ? (const T *)m->mut_f.p : (const T *)m->mut_d.p;
+            s.mus = sizeof(T) == 4 ? (const T *)m->mus_f.p : (const T *)m->mus_d.p;
+            s.g = T(m->g);
+            set_pixels(s, ctx->ws_pix.as<TA>() + (size_t)kPixBufs * P * q, P);
+            s.red = ctx->ws_red.as<TA>() + 2 * q;
+            s.mode = 1;
+            s.kim = kim ? T(kim[i]) : T(0);
+        }
         hs[i] = HostSolve{m, 0, 0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0};
         if (rs) continue;
         // beam columns and power per column (CPU package/src/Dist.c:1090-1114, Inits.c:56)
@@ -835,17 +860,17 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
             }
         if (cols == 0) return fail(RTE_E_ARG, "solve %d: beam hits no voxel column", i);
         bd[i] = BeamDev{beams[i].pos_x, beams[i].pos_y, half, beams[i].mag / (double)cols,
-                        m->mua_d.as<double>(), m->mus_d.as<double>()};
+                        m->mua_d.as<double>(), m->mus_d.as<double>(), kim ? kim[i] : 0.0};
     }
-    // the final field sum_q A W_q of solve 0, if asked for
+    // the final field sum_q A W_q of solve 0 (both halves when complex), if asked for
     DevBuf dphi;
     if (field) {
-        HIP_TRY(dphi.alloc(szW * sizeof(TA)));
-        HIP_TRY(launch_fill<TA>(dphi.as<TA>(), szW, TA(0), st));
-        sv[0].phi = dphi.as<TA>();
+        HIP_TRY(dphi.alloc(szW * hv * sizeof(TA)));
+        HIP_TRY(launch_fill<TA>(dphi.as<TA>(), szW * hv, TA(0), st));
+        for (int hh = 0; hh < hv; ++hh) sv[hh].phi = dphi.as<TA>() + szW * hh;
     }
     SolveDev<T, TA> *svd = ctx->ws_sv.as<SolveDev<T, TA>>();
-    HIP_TRY(hipMemcpyAsync(svd, sv.data(), sizeof(SolveDev<T, TA>) * nb, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(svd, sv.data(), sizeof(SolveDev<T, TA>) * ns, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(ctx->ws_beams.p, bd.data(), sizeof(BeamDev) * nb, hipMemcpyHostToDevice, st));
     const Scales sc = make_scales(L);
     {
@@ -858,24 +883,30 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
                 return fail(RTE_E_UNSUPP, "derivative solves need FP64 running sums");
             }
         } else {
-            HIP_TRY(launch_fill<T>(ctx->ws_W.as<T>(), szW * nb, T(0), st));
-            HIP_TRY(launch_source<T, TA>(L, g, dx, dy, dz, grid.xmin, grid.ymin, sc, ctx->ws_beams.as<BeamDev>(),
-                                         svd, nb, st));
+            HIP_TRY(launch_fill<T>(ctx->ws_W.as<T>(), szW * ns, T(0), st));
+            if (kim)
+                HIP_TRY(launch_source_c<T, TA>(L, g, dx, dy, dz, grid.xmin, grid.ymin, sc,
+                                               ctx->ws_beams.as<BeamDev>(), svd, nb, st));
+            else
+                HIP_TRY(launch_source<T, TA>(L, g, dx, dy, dz, grid.xmin, grid.ymin, sc,
+                                             ctx->ws_beams.as<BeamDev>(), svd, nb, st));
         }
-        HIP_TRY(launch_bottom<T, TA>(P, svd, nb, st));
+        HIP_TRY(launch_bottom<T, TA>(P, svd, ns, st));
         tm.done();
     }
-    ctx->host_red.resize(sizeof(TA) * 2 * nb);
+    ctx->host_red.resize(sizeof(TA) * 2 * ns);
     TA *red = reinterpret_cast<TA *>(ctx->host_red.data());
-    HIP_TRY(hipMemcpyAsync(red, ctx->ws_red.p, sizeof(TA) * 2 * nb, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(red, ctx->ws_red.p, sizeof(TA) * 2 * ns, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     for (int i = 0; i < nb; ++i) {
         HostSolve &h = hs[i];
-        h.ball = (double)red[2 * i];
+        const int q = i * hv;
+        h.ball = (double)red[2 * q];
         h.T = h.ball;
+        h.Ti = hv == 2 ? (double)red[2 * q + 2] : 0.0;
         if (p->stop_rule == RTE_STOP_LAMBDA) {
             h.n = 1;
-            h.sumOut = (double)red[2 * i + 1];
+            h.sumOut = (double)red[2 * q + 1] + (hv == 2 ? (double)red[2 * q + 3] : 0.0);
             h.state = (max_terms <= 1) ? 2 : 0;
         } else {
             h.n = 0;
@@ -905,7 +936,10 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
             ca.coefE[l] = TA(cE[l]);
             ca.coefR[l] = TA(cR[l]);
         }
-        for (int a = 0; a < K; ++a) ca.wsum[a] = TA(stop_weight(a));
+        for (int a = 0; a < K; ++a) {
+            ca.wsum[a] = TA(stop_weight(a));
+            ca.wsumI[a] = TA(stop_weight_imag(a));
+        }
         for (int a = 0; a < kMaxK; ++a) ca.nsc[a] = TA(sc.n[a]);
     }
 
@@ -917,14 +951,19 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
         for (int i = 0; i < nb; ++i)
             if (hs[i].state != 3) act.push_back(i);
         if (act.empty()) break;
-        svact.resize(act.size());
-        for (size_t j = 0; j < act.size(); ++j) {
-            svact[j] = sv[act[j]];
-            svact[j].mode = hs[act[j]].state;
-        }
+        svact.resize(act.size() * hv);
+        for (size_t j = 0; j < act.size(); ++j)
+            for (int hh = 0; hh < hv; ++hh) {
+                svact[j * hv + hh] = sv[act[j] * hv + hh];
+                svact[j * hv + hh].mode = hs[act[j]].state;
+            }
         const int na = (int)act.size();
-        HIP_TRY(hipMemcpyAsync(svd, svact.data(), sizeof(SolveDev<T, TA>) * na, hipMemcpyHostToDevice, st));
-        if (!multi) {
+        HIP_TRY(hipMemcpyAsync(svd, svact.data(), sizeof(SolveDev<T, TA>) * na * hv, hipMemcpyHostToDevice, st));
+        if (kim) {
+            LaunchTimer tm(ctx, 0);
+            HIP_TRY(launch_prop_c<T, TA>(L, path, pa, svd, na, st));
+            tm.done(cp_pairs * 2 * na, (flops_dir + 2 * flops_rhs) * na);
+        } else if (!multi) {
             LaunchTimer tm(ctx, 0);
             HIP_TRY(launch_prop<T, TA>(L, path, 1, pa, svd, na, st));
             tm.done(cp_pairs * na, (flops_dir + flops_rhs) * na);
@@ -965,44 +1004,52 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
         }
         {
             LaunchTimer tm(ctx, 2);
-            HIP_TRY(launch_combine<T, TA>(L, ca, svd, na, st));
-            HIP_TRY(launch_bottom<T, TA>(P, svd, na, st));
+            if (kim)
+                HIP_TRY(launch_combine_c<T, TA>(L, ca, svd, na, st));
+            else
+                HIP_TRY(launch_combine<T, TA>(L, ca, svd, na, st));
+            HIP_TRY(launch_bottom<T, TA>(P, svd, na * hv, st));
             tm.done();
         }
-        std::vector<TA> r(2 * nb);
-        HIP_TRY(hipMemcpyAsync(r.data(), ctx->ws_red.p, sizeof(TA) * 2 * nb, hipMemcpyDeviceToHost, st));
+        std::vector<TA> r(2 * ns);
+        HIP_TRY(hipMemcpyAsync(r.data(), ctx->ws_red.p, sizeof(TA) * 2 * ns, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         const double now_ms =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
         for (int j = 0; j < na; ++j) {
             HostSolve &h = hs[act[j]];
-            const double dT = (double)r[2 * act[j]], sumW = (double)r[2 * act[j] + 1];
+            const int q = act[j] * hv;
+            const double dT = (double)r[2 * q];
+            const double sumW = (double)r[2 * q + 1] + (hv == 2 ? (double)r[2 * q + 3] : 0.0);
+            const double dTi = hv == 2 ? (double)r[2 * q + 2] : 0.0;
             lat[act[j]] = now_ms;
-            advance_series(h, dT, sumW, p->stop_rule, p->tol, max_terms);
+            advance_series(h, dT, sumW, p->stop_rule, p->tol, max_terms, dTi);
         }
         // the rows of svd hold the active solves; stale entries are rebuilt next pass
     }
     HIP_TRY(hipEventRecord(ctx->ev1, st));
-    // gather exitance images
+    // gather exitance images (a complex solve: real plane, then imaginary plane)
     std::vector<TA> img((size_t)P);
-    for (int i = 0; i < nb; ++i) {
-        HIP_TRY(hipMemcpyAsync(img.data(), sv[i].E, sizeof(TA) * P, hipMemcpyDeviceToHost, st));
+    for (int q0 = 0; q0 < ns; ++q0) {
+        HIP_TRY(hipMemcpyAsync(img.data(), sv[q0].E, sizeof(TA) * P, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
-        for (int q = 0; q < P; ++q) exitance[(size_t)i * P + q] = (double)img[q];
+        for (int q = 0; q < P; ++q) exitance[(size_t)q0 * P + q] = (double)img[q];
         if (reflectance) {
-            HIP_TRY(hipMemcpyAsync(img.data(), sv[i].R, sizeof(TA) * P, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipMemcpyAsync(img.data(), sv[q0].R, sizeof(TA) * P, hipMemcpyDeviceToHost, st));
             HIP_TRY(hipStreamSynchronize(st));
-            for (int q = 0; q < P; ++q) reflectance[(size_t)i * P + q] = (double)img[q];
+            for (int q = 0; q < P; ++q) reflectance[(size_t)q0 * P + q] = (double)img[q];
         }
     }
     if (field) {
         // coefficient-major, in the reference's units (A carries 1/c_m there)
-        std::vector<TA> f(szW);
-        HIP_TRY(hipMemcpyAsync(f.data(), dphi.p, sizeof(TA) * szW, hipMemcpyDeviceToHost, st));
+        std::vector<TA> f(szW * hv);
+        HIP_TRY(hipMemcpyAsync(f.data(), dphi.p, sizeof(TA) * szW * hv, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         const double inv_cm = media[0]->n / media[0]->c;
-        for (int v = 0; v < N; ++v)
-            for (int a = 0; a < K; ++a) field[(size_t)a * N + v] = (double)f[(size_t)v * K + a] * inv_cm;
+        for (int hh = 0; hh < hv; ++hh)
+            for (int v = 0; v < N; ++v)
+                for (int a = 0; a < K; ++a)
+                    field[szW * hh + (size_t)a * N + v] = (double)f[szW * hh + (size_t)v * K + a] * inv_cm;
     }
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -1095,6 +1142,42 @@ extern "C" int rte_solve_field(rte_ctx *ctx, rte_medium *m, const rte_beam *beam
         return run_batch<float, double>(ctx, 1, media, beam, params, exitance, stats, nullptr, reflectance, field);
     default:
         return run_batch<float, float>(ctx, 1, media, beam, params, exitance, stats, nullptr, reflectance, field);
+    }
+}
+
+extern "C" int rte_solve_fd(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_beam *beams,
+                            const double *omega, const rte_params *params, double *exitance, double *reflectance,
+                            double *field, rte_stats *stats)
+{
+    if (!ctx || !media || !beams || !omega || !exitance) return fail(RTE_E_ARG, "rte_solve_fd: NULL argument");
+    if (nb < 1) return fail(RTE_E_ARG, "rte_solve_fd: nb < 1");
+    int rc;
+    if ((rc = check_params(params))) return rc;
+    std::vector<double> kim(nb);
+    for (int i = 0; i < nb; ++i) {
+        if (!media[i]) return fail(RTE_E_ARG, "media[%d] is NULL", i);
+        if (media[i]->ctx != ctx) return fail(RTE_E_ARG, "media[%d] belongs to another context", i);
+        const rte_grid &a = media[i]->grid, &b = media[0]->grid;
+        if (a.nx != b.nx || a.ny != b.ny || a.nz != b.nz || a.xmin != b.xmin || a.xmax != b.xmax ||
+            a.ymin != b.ymin || a.ymax != b.ymax || a.zmin != b.zmin || a.zmax != b.zmax)
+            return fail(RTE_E_ARG, "media[%d] has a different grid than media[0]", i);
+        if (!std::isfinite(beams[i].pos_x) || !std::isfinite(beams[i].pos_y) || !(beams[i].diam >= 0.0))
+            return fail(RTE_E_ARG, "beam %d invalid", i);
+        if (!std::isfinite(omega[i])) return fail(RTE_E_ARG, "omega[%d] is not finite", i);
+        // CPU package/src/RTE_uniform.c:126: mu_a + i modulation / (c / n)
+        kim[i] = omega[i] * media[i]->n / media[i]->c;
+    }
+    if ((rc = set_device(ctx))) return rc;
+    switch (params->precision) {
+    case RTE_PREC_FP64:
+        return run_batch<double, double>(ctx, nb, media, beams, params, exitance, stats, nullptr, reflectance, field,
+                                         kim.data());
+    case RTE_PREC_MIXED:
+        return run_batch<float, double>(ctx, nb, media, beams, params, exitance, stats, nullptr, reflectance, field,
+                                        kim.data());
+    default:
+        return run_batch<float, float>(ctx, nb, media, beams, params, exitance, stats, nullptr, reflectance, field,
+                                       kim.data());
     }
 }
 

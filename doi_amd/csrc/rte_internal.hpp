@@ -57,6 +57,7 @@ struct SolveDev {
     TA *aw_store;      // [N][K] if set, k_combine also writes A W (unscaled) here
     TA *phi;           // [N][K] if set, k_combine adds A W (unscaled) here: the final field
     int mode;          // 0 term, 1 init, 2 final, 3 recompute (see k_bottom / k_combine)
+    T kim;             // frequency-domain solves: Im mu_t = omega n / c (uniform), 0 otherwise
 };
 
 template <typename T>
@@ -81,6 +82,7 @@ struct CombArgs {
     TA coefE[kMaxL + 1];  // exitance weight of w_l0 (bottom face)
     TA coefR[kMaxL + 1];  // reflectance weight of w_l0 (top face)
     TA wsum[kMaxK];       // stop-rule weight of coefficient a
+    TA wsumI[kMaxK];      // the same for the imaginary half of a complex field
     TA nsc[kMaxK];        // n_a of the scaled coefficients
     TA *aw_out;           // op = 1
 };
@@ -93,6 +95,7 @@ struct Scales {
 struct BeamDev {
     double pos_x, pos_y, half_diam, fact;  // fact = power per beam column
     const double *mua, *mus;               // FP64 maps
+    double kim;                            // Im mu_t of a frequency-domain solve (0: continuous wave)
 };
 
 // launchers (rte_kernels.hip)
@@ -102,17 +105,29 @@ hipError_t launch_fill(T *p, size_t n, T v, hipStream_t st);
 template <typename T, typename TA>
 hipError_t launch_prop(int L, int path, int nr, const PropArgs<T> &A, const SolveDev<T, TA> *sv, int ngroups,
                        hipStream_t st);
+// frequency-domain solves: group z = (sv[2z], sv[2z+1]) = (real, imaginary) half of one complex field
+template <typename T, typename TA>
+hipError_t launch_prop_c(int L, int path, const PropArgs<T> &A, const SolveDev<T, TA> *sv, int ncomplex,
+                         hipStream_t st);
 template <typename T>
 hipError_t launch_ktab(const GridDev<T> &g, T thr2, const T *mut, T *ktab, hipStream_t st);
 template <typename T, typename TA>
 hipError_t launch_combine(int L, const CombArgs<T, TA> &A, const SolveDev<T, TA> *sv, int nsolve,
                           hipStream_t st);
 template <typename T, typename TA>
+hipError_t launch_combine_c(int L, const CombArgs<T, TA> &A, const SolveDev<T, TA> *sv, int ncomplex,
+                            hipStream_t st);
+template <typename T, typename TA>
 hipError_t launch_bottom(int P, const SolveDev<T, TA> *sv, int nsolve, hipStream_t st);
 template <typename T, typename TA>
 hipError_t launch_source(int L, const GridDev<T> &g, double dx, double dy, double dz, double xmin,
                          double ymin, const Scales &sc, const BeamDev *beams, const SolveDev<T, TA> *sv,
                          int nsolve, hipStream_t st);
+// complex source of ncomplex frequency-domain solves (halves as in launch_prop_c)
+template <typename T, typename TA>
+hipError_t launch_source_c(int L, const GridDev<T> &g, double dx, double dy, double dz, double xmin,
+                           double ymin, const Scales &sc, const BeamDev *beams, const SolveDev<T, TA> *sv,
+                           int ncomplex, hipStream_t st);
 hipError_t launch_moments(int K, const double *R, const double *sn, const double *beta, double *M,
                           hipStream_t st);
 // in: coefficient-major double [K][N] -> out: voxel-major T [N][K], times sc.n[a]

@@ -38,9 +38,10 @@ __global__ void k_fill(T *__restrict__ p, size_t n, T v)
 //   and applied to every right-hand side.
 // PATH 0: kf from the per-medium table; PATH 1: marched per pair.
 // ---------------------------------------------------------------------------
-template <typename T, typename TA, int L, int PATH, int NR>
+template <typename T, typename TA, int L, int PATH, int NR, bool CPLX = false>
 __global__ __launch_bounds__(kWave) void k_prop(PropArgs<T> A, const SolveDev<T, TA> *__restrict__ sv)
 {
+    static_assert(!CPLX || NR == 2, "a complex field is propagated as its two real halves");
     constexpr int K = (L + 1) * (L + 1);
     const GridDev<T> &g = A.g;
     const int lane = threadIdx.x, blk = blockIdx.x, chunk = blockIdx.y;
@@ -65,6 +66,7 @@ __global__ __launch_bounds__(kWave) void k_prop(PropArgs<T> A, const SolveDev<T,
     }
     const gptr<T> ktab = as_global<T>(grp[0].ktab);
     const gptr<T> mut = as_global<T>(grp[0].mut);
+    const T kim = grp[0].kim;
     // destination coordinates; source coordinates advance by dx along x
     const T dxc = T(cdi) * g.dx, dyc = T(cdj) * g.dy, dzc = T(cdk) * g.dz;
     int si = s0 % g.nx, sj = (s0 / g.nx) % g.ny, sk = s0 / nxy;
@@ -97,15 +99,36 @@ __global__ __launch_bounds__(kWave) void k_prop(PropArgs<T> A, const SolveDev<T,
         // w holds right-hand side r of source s; the next one (r+1, or 0 of
         // s+1) is requested before r is applied, hiding the scalar-load latency
         const int snext = (s + 1 < s1) ? s + 1 : s;
-#pragma unroll
-        for (int r = 0; r < NR; ++r) {
+        if constexpr (CPLX) {
+            // the real and imaginary halves of one complex field share the pair:
+            // t = kf e^{-i kim r} (p0 + i p1), both projections taken first
             T wn[K];
 #pragma unroll
-            for (int a = 0; a < K; ++a)
-                wn[a] = (r + 1 < NR) ? Wsrc[r + 1][(size_t)s * K + a] : Wsrc[0][(size_t)snext * K + a];
-            pair_apply<T, L>(acc[r], w, kf, G);
+            for (int a = 0; a < K; ++a) wn[a] = Wsrc[1][(size_t)s * K + a];
+            const T p0 = pair_dot<T, L>(w, G);
 #pragma unroll
             for (int a = 0; a < K; ++a) w[a] = wn[a];
+#pragma unroll
+            for (int a = 0; a < K; ++a) wn[a] = Wsrc[0][(size_t)snext * K + a];
+            const T p1 = pair_dot<T, L>(w, G);
+#pragma unroll
+            for (int a = 0; a < K; ++a) w[a] = wn[a];
+            T sn, cs;
+            dev_sincos<T>(kim * (r2 * rinv), &sn, &cs);
+            const T kc = kf * cs, ks = kf * sn;
+            pair_axpy<T, L>(acc[0], kc * p0 + ks * p1, G);
+            pair_axpy<T, L>(acc[1], kc * p1 - ks * p0, G);
+        } else {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                T wn[K];
+#pragma unroll
+                for (int a = 0; a < K; ++a)
+                    wn[a] = (r + 1 < NR) ? Wsrc[r + 1][(size_t)s * K + a] : Wsrc[0][(size_t)snext * K + a];
+                pair_apply<T, L>(acc[r], w, kf, G);
+#pragma unroll
+                for (int a = 0; a < K; ++a) w[a] = wn[a];
+            }
         }
         if (++si == g.nx) {
             si = 0;
@@ -215,6 +238,10 @@ __global__ __launch_bounds__(1024) void k_ktab(GridDev<T> g, int ntiles, T thr2,
 // grid = (ceil(N/64), nsolve), block = 64.
 // ---------------------------------------------------------------------------
 template <typename T, typename TA, int L>
+__device__ __forceinline__ void combine_out(const CombArgs<T, TA> &A, const SolveDev<T, TA> &S, int v,
+                                            const TA (&acc)[(L + 1) * (L + 1)], const TA *__restrict__ wsum);
+
+template <typename T, typename TA, int L>
 __global__ __launch_bounds__(kWave) void k_combine(CombArgs<T, TA> A, const SolveDev<T, TA> *__restrict__ sv)
 {
     constexpr int K = (L + 1) * (L + 1);
@@ -269,15 +296,26 @@ __global__ __launch_bounds__(kWave) void k_combine(CombArgs<T, TA> A, const Solv
     }
 #pragma unroll
     for (int a = 0; a < K; ++a) acc[a] *= A.dV;
-
-    const int zk = v / (g.nx * g.ny);
-    const int pix = v - zk * g.nx * g.ny;
-    const bool bottom = (zk == g.nz - 1);
     if (A.op) {
 #pragma unroll
         for (int a = 0; a < K; ++a) A.aw_out[(size_t)v * K + a] = acc[a];
         return;
     }
+    combine_out<T, TA, L>(A, S, v, acc, A.wsum);
+}
+
+// The per-solve outputs of one voxel's A W (acc, unscaled, TA): stored or
+// accumulated copies, the exitance and reflectance increments, and in a term
+// the scattered next term with its stop-rule sums (weights wsum).
+template <typename T, typename TA, int L>
+__device__ __forceinline__ void combine_out(const CombArgs<T, TA> &A, const SolveDev<T, TA> &S, int v,
+                                            const TA (&acc)[(L + 1) * (L + 1)], const TA *__restrict__ wsum)
+{
+    constexpr int K = (L + 1) * (L + 1);
+    const GridDev<T> &g = A.g;
+    const int zk = v / (g.nx * g.ny);
+    const int pix = v - zk * g.nx * g.ny;
+    const bool bottom = (zk == g.nz - 1);
     if (S.aw_store) {
 #pragma unroll
         for (int a = 0; a < K; ++a) S.aw_store[(size_t)v * K + a] = acc[a];
@@ -309,11 +347,88 @@ __global__ __launch_bounds__(kWave) void k_combine(CombArgs<T, TA> A, const Solv
                 const TA wn = gl * ms * acc[a];
                 const T wt = T(wn * A.nsc[a]);
                 S.W[(size_t)v * K + a] = wt;
-                bs += A.wsum[a] * (TA(wt) / A.nsc[a]);
+                bs += wsum[a] * (TA(wt) / A.nsc[a]);
             }
             gl *= TA(S.g);
         }
         if (bottom && S.mode == 0) S.bsum[pix] = bs;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Combine of frequency-domain solves: halves (sv[2y], sv[2y+1]) = (real,
+// imaginary) of one complex field.  The self block has the complex series
+// coefficients c_k = (-mu)^{k-1}/k!, mu = mu_t + i kim:
+//   S (w0 + i w1) = sum_k (Re c_k M_k w0 - Im c_k M_k w1) + i (Re c_k M_k w1 + Im c_k M_k w0);
+// everything else acts on each half alone (D, the exitance and reflectance
+// functionals and the chunk sums are real).  Both halves' W are read before
+// either is overwritten by the scattered next term.
+// grid = (ceil(N/64), ncomplex), block = 64.
+// ---------------------------------------------------------------------------
+template <typename T, typename TA, int L>
+__global__ __launch_bounds__(kWave) void k_combine_c(CombArgs<T, TA> A, const SolveDev<T, TA> *__restrict__ sv)
+{
+    constexpr int K = (L + 1) * (L + 1);
+    const GridDev<T> &g = A.g;
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const SolveDev<T, TA> S0 = sv[2 * blockIdx.y], S1 = sv[2 * blockIdx.y + 1];
+    if (v >= g.N) return;
+    T w0[K], w1[K], st0[K], st1[K];
+#pragma unroll
+    for (int a = 0; a < K; ++a) {
+        w0[a] = T(TA(S0.W[(size_t)v * K + a]) / A.nsc[a]);
+        w1[a] = T(TA(S1.W[(size_t)v * K + a]) / A.nsc[a]);
+        st0[a] = st1[a] = T(0);
+    }
+    const T mr = S0.mut[v], mi = S0.kim;
+    T cr = T(1), ci = T(0);
+    for (int k = 0; k < kNMom; ++k) {
+        if (k > 0) {
+            const T nr = -(cr * mr - ci * mi) / T(k + 1);
+            const T ni = -(cr * mi + ci * mr) / T(k + 1);
+            cr = nr;
+            ci = ni;
+        }
+        if constexpr (sizeof(T) == 4) {
+            const T *__restrict__ M = A.momT + (size_t)k * K * K;
+            self_apply<K>(st0, cr, M, w0, std::make_integer_sequence<int, K>{});
+            self_apply<K>(st0, -ci, M, w1, std::make_integer_sequence<int, K>{});
+            self_apply<K>(st1, cr, M, w1, std::make_integer_sequence<int, K>{});
+            self_apply<K>(st1, ci, M, w0, std::make_integer_sequence<int, K>{});
+        } else {
+            const TA *__restrict__ M = A.mom + (size_t)k * K * K;
+#pragma unroll
+            for (int a = 0; a < K; ++a) {
+                T s0 = T(0), s1 = T(0);
+#pragma unroll
+                for (int b = 0; b < K; ++b) {
+                    s0 += T(M[a * K + b]) * w0[b];
+                    s1 += T(M[a * K + b]) * w1[b];
+                }
+                st0[a] += cr * s0 - ci * s1;
+                st1[a] += cr * s1 + ci * s0;
+            }
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const SolveDev<T, TA> &S = h ? S1 : S0;
+        TA acc[K];
+#pragma unroll
+        for (int a = 0; a < K; ++a) acc[a] = TA(0);
+        for (int c = 0; c < A.nchunk; ++c) {
+            const T *__restrict__ p = S.part + ((size_t)c * g.N + v) * K;
+#pragma unroll
+            for (int a = 0; a < K; ++a) acc[a] += TA(p[a]);
+        }
+        // the steps of k_combine, in its order
+#pragma unroll
+        for (int a = 0; a < K; ++a) acc[a] *= A.nsc[a];
+#pragma unroll
+        for (int a = 0; a < K; ++a) acc[a] += A.self_scale * TA(h ? st1[a] : st0[a]);
+#pragma unroll
+        for (int a = 0; a < K; ++a) acc[a] *= A.dV;
+        combine_out<T, TA, L>(A, S, v, acc, h ? A.wsumI : A.wsum);
     }
 }
 
@@ -409,6 +524,80 @@ __global__ __launch_bounds__(kWave) void k_source(GridDev<T> g, double dxd, doub
     S.dE[pix] = TA(0);
     S.R[pix] = TA(0);
     S.dR[pix] = TA(0);
+}
+
+// ---------------------------------------------------------------------------
+// Complex first-scatter source and ballistic image of frequency-domain solves:
+// k_source with mu_t -> mu_t + i kim (the column attenuation e^{-mu_t dz}
+// picks up the phase e^{-i kim dz}, and mu_s/mu_t is a complex ratio); real
+// part to sv[2y], imaginary part to sv[2y+1].  W must be zero.
+// grid = (ceil(P/64), ncomplex), block = 64.
+// ---------------------------------------------------------------------------
+template <typename T, typename TA, int L>
+__global__ __launch_bounds__(kWave) void k_source_c(GridDev<T> g, double dxd, double dyd, double dzd,
+                                                   double xmin, double ymin, Scales sc,
+                                                   const BeamDev *__restrict__ beams,
+                                                   const SolveDev<T, TA> *__restrict__ sv)
+{
+    constexpr int K = (L + 1) * (L + 1);
+    const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int P = g.nx * g.ny;
+    if (pix >= P) return;
+    const BeamDev B = beams[blockIdx.y];
+    const SolveDev<T, TA> S0 = sv[2 * blockIdx.y], S1 = sv[2 * blockIdx.y + 1];
+    const int i = pix % g.nx, j = pix / g.nx;
+    const double xc = (double)i * dxd + xmin + dxd / 2.0;
+    const double yc = (double)j * dyd + ymin + dyd / 2.0;
+    const double rad = sqrt((xc - B.pos_x) * (xc - B.pos_x) + (yc - B.pos_y) * (yc - B.pos_y));
+    const bool in = rad <= B.half_diam;
+    const double dV = dxd * dyd * dzd;
+    const double gg = (double)S0.g;
+    const double ki = B.kim;
+    const double ph_c = cos(ki * dzd), ph_s = sin(ki * dzd);
+    double Tr = 1.0, Ti = 0.0;
+    double bs = 0.0;
+    if (in) {
+        for (int k = 0; k < g.nz; ++k) {
+            const int v = pix + P * k;
+            const double mt = B.mua[v] + B.mus[v];
+            const double e = exp(-mt * dzd);
+            const double ar = e * ph_c, ai = -e * ph_s;               // e^{-(mt + i ki) dz}
+            // mu_s / (mt + i ki) = (mu_s/mt) / (1 + i t), t = ki/mt; the products in
+            // k_source's order, so that ki = 0 gives its bits
+            const double t = ki / mt, q0 = B.mus[v] / mt, d = 1.0 + t * t;
+            const double fr = B.fact * (q0 / d), fi = B.fact * (-q0 * t / d);
+            const double ftr = fr * Tr - fi * Ti, fti = fr * Ti + fi * Tr;
+            const double omr = 1.0 - ar, omi = -ai;
+            const double br = (ftr * omr - fti * omi) / dV;
+            const double bi = (ftr * omi + fti * omr) / dV;
+            double gl = 1.0;
+#pragma unroll
+            for (int l = 0; l <= L; ++l) {
+                const double f = gl * sqrt((2.0 * l + 1.0) / (4.0 * M_PI));
+                const int a = l * l + l;
+                const T sr = T(f * br * sc.n[a]);
+                S0.W[(size_t)v * K + a] = sr;
+                S1.W[(size_t)v * K + a] = T(f * bi * sc.n[a]);
+                if (k == g.nz - 1) bs += (double)sr / sc.n[a];
+                gl *= gg;
+            }
+            const double nTr = Tr * ar - Ti * ai, nTi = Tr * ai + Ti * ar;
+            Tr = nTr;
+            Ti = nTi;
+        }
+    }
+    S0.E[pix] = TA(in ? B.fact * Tr : 0.0);
+    S1.E[pix] = TA(in ? B.fact * Ti : 0.0);
+    // the stop-rule weights of the imaginary half vanish on m = 0
+    S0.bsum[pix] = TA(bs);
+    S1.bsum[pix] = TA(0);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const SolveDev<T, TA> &S = h ? S1 : S0;
+        S.dE[pix] = TA(0);
+        S.R[pix] = TA(0);
+        S.dR[pix] = TA(0);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -602,6 +791,20 @@ hipError_t launch_prop(int L, int path, int nr, const PropArgs<T> &A, const Solv
     return hipGetLastError();
 }
 
+template <typename T, typename TA>
+hipError_t launch_prop_c(int L, int path, const PropArgs<T> &A, const SolveDev<T, TA> *sv, int ncomplex,
+                         hipStream_t st)
+{
+    dim3 grid(A.g.nblk, A.nchunk, ncomplex), block(kWave);
+    RTE_DISPATCH_L(L, {
+        if (path == 1)
+            hipLaunchKernelGGL((k_prop<T, TA, LL, 1, 2, true>), grid, block, 0, st, A, sv);
+        else
+            hipLaunchKernelGGL((k_prop<T, TA, LL, 0, 2, true>), grid, block, 0, st, A, sv);
+    });
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_ktab(const GridDev<T> &g, T thr2, const T *mut, T *ktab, hipStream_t st)
 {
@@ -642,6 +845,15 @@ hipError_t launch_combine(int L, const CombArgs<T, TA> &A, const SolveDev<T, TA>
 }
 
 template <typename T, typename TA>
+hipError_t launch_combine_c(int L, const CombArgs<T, TA> &A, const SolveDev<T, TA> *sv, int ncomplex,
+                            hipStream_t st)
+{
+    dim3 grid((A.g.N + kWave - 1) / kWave, ncomplex), block(kWave);
+    RTE_DISPATCH_L(L, hipLaunchKernelGGL((k_combine_c<T, TA, LL>), grid, block, 0, st, A, sv));
+    return hipGetLastError();
+}
+
+template <typename T, typename TA>
 hipError_t launch_bottom(int P, const SolveDev<T, TA> *sv, int nsolve, hipStream_t st)
 {
     hipLaunchKernelGGL((k_bottom<T, TA>), dim3(nsolve), dim3(256), 0, st, P, sv);
@@ -655,6 +867,17 @@ hipError_t launch_source(int L, const GridDev<T> &g, double dx, double dy, doubl
 {
     dim3 grid((g.nx * g.ny + kWave - 1) / kWave, nsolve), block(kWave);
     RTE_DISPATCH_L(L, hipLaunchKernelGGL((k_source<T, TA, LL>), grid, block, 0, st, g, dx, dy, dz,
+                                         xmin, ymin, sc, beams, sv));
+    return hipGetLastError();
+}
+
+template <typename T, typename TA>
+hipError_t launch_source_c(int L, const GridDev<T> &g, double dx, double dy, double dz, double xmin,
+                           double ymin, const Scales &sc, const BeamDev *beams, const SolveDev<T, TA> *sv,
+                           int ncomplex, hipStream_t st)
+{
+    dim3 grid((g.nx * g.ny + kWave - 1) / kWave, ncomplex), block(kWave);
+    RTE_DISPATCH_L(L, hipLaunchKernelGGL((k_source_c<T, TA, LL>), grid, block, 0, st, g, dx, dy, dz,
                                          xmin, ymin, sc, beams, sv));
     return hipGetLastError();
 }
@@ -709,6 +932,27 @@ hipError_t launch_tau_list(const GridDev<float> &g, const float *mut, int np, co
 }
 
 // explicit instantiations used by the host library
+template hipError_t launch_prop_c<float, double>(int, int, const PropArgs<float> &, const SolveDev<float, double> *, int,
+                                                     hipStream_t);
+template hipError_t launch_combine_c<float, double>(int, const CombArgs<float, double> &, const SolveDev<float, double> *, int,
+                                                        hipStream_t);
+template hipError_t launch_source_c<float, double>(int, const GridDev<float> &, double, double, double, double, double,
+                                                       const Scales &, const BeamDev *, const SolveDev<float, double> *, int,
+                                                       hipStream_t);
+template hipError_t launch_prop_c<double, double>(int, int, const PropArgs<double> &, const SolveDev<double, double> *, int,
+                                                     hipStream_t);
+template hipError_t launch_combine_c<double, double>(int, const CombArgs<double, double> &, const SolveDev<double, double> *, int,
+                                                        hipStream_t);
+template hipError_t launch_source_c<double, double>(int, const GridDev<double> &, double, double, double, double, double,
+                                                       const Scales &, const BeamDev *, const SolveDev<double, double> *, int,
+                                                       hipStream_t);
+template hipError_t launch_prop_c<float, float>(int, int, const PropArgs<float> &, const SolveDev<float, float> *, int,
+                                                     hipStream_t);
+template hipError_t launch_combine_c<float, float>(int, const CombArgs<float, float> &, const SolveDev<float, float> *, int,
+                                                        hipStream_t);
+template hipError_t launch_source_c<float, float>(int, const GridDev<float> &, double, double, double, double, double,
+                                                       const Scales &, const BeamDev *, const SolveDev<float, float> *, int,
+                                                       hipStream_t);
 template hipError_t launch_fill<float>(float *, size_t, float, hipStream_t);
 template hipError_t launch_fill<double>(double *, size_t, double, hipStream_t);
 template hipError_t launch_prop<float, double>(int, int, int, const PropArgs<float> &,

@@ -103,6 +103,22 @@ def write_image(path, real, imag=None):
         f.write(np.ascontiguousarray(im).astype("<f8").tobytes())
 
 
+def write_phase_image(path, phase):
+    """OutputPhaseImage: int32 nX, int32 nY, one plane of doubles, x fastest
+    (CPU package/src/Dist.c:46-65)."""
+    phase = np.asarray(phase, dtype=np.float64)
+    ny, nx = phase.shape
+    with open(path, "wb") as f:
+        f.write(struct.pack("<ii", nx, ny))
+        f.write(np.ascontiguousarray(phase).astype("<f8").tobytes())
+
+
+def read_phase_image(path):
+    raw = open(path, "rb").read()
+    nx, ny = struct.unpack("<ii", raw[:8])
+    return np.frombuffer(raw[8:8 + 8 * nx * ny], dtype="<f8").reshape(ny, nx).copy()
+
+
 def read_image(path):
     """(real, imag) planes of an OutputImage file, each [ny][nx]."""
     raw = open(path, "rb").read()

@@ -105,6 +105,8 @@ SIGNATURES = {
     "rte_solve_images": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(Beam), C.POINTER(Params),
                                    _dp, _dp, C.POINTER(Stats)]),
     "rte_solve_field": (C.c_int, [_vp, _vp, C.POINTER(Beam), C.POINTER(Params), _dp, _dp, _dp, C.POINTER(Stats)]),
+    "rte_solve_fd": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(Beam), _dp, C.POINTER(Params), _dp, _dp, _dp,
+                               C.POINTER(Stats)]),
     "rte_exitance_vjp": (C.c_int, [_vp, _vp, C.POINTER(Beam), C.POINTER(Params), C.POINTER(GradOpts), _dp, _dp,
                                    _dp, _dp, C.POINTER(Stats)]),
     "rte_gradient": (C.c_int, [_vp, _vp, C.POINTER(Beam), C.POINTER(Params), C.POINTER(GradOpts), _dp, _dp, _dp,
@@ -271,6 +273,30 @@ class Context:
         _check(lib().rte_solve_field(self.h, medium.h, C.byref(Beam(*beam)), C.byref(params), _p(field), _p(img),
                                      _p(refl), C.byref(st)))
         return field.reshape(K, medium.N), img.reshape(ny, nx), refl.reshape(ny, nx), _stats_dict(st)
+
+    def solve_fd(self, media, beams, omegas, params, field=False):
+        """Frequency-domain solves (rte_solve_fd): complex transmitted and
+        reflectance images [nb, ny, nx], stats, and with field=True the final
+        complex field [K, N] of solve 0."""
+        nb = len(media)
+        nx, ny, _ = media[0].shape
+        K = (params.L + 1) ** 2
+        arr = (_vp * nb)(*[m.h for m in media])
+        barr = (Beam * nb)(*[Beam(*b) for b in beams])
+        om = _f64(np.broadcast_to(np.asarray(omegas, dtype=np.float64), (nb,)))
+        img = np.zeros(nb * 2 * nx * ny)
+        refl = np.zeros(nb * 2 * nx * ny)
+        fld = np.zeros(2 * K * media[0].N) if field else None
+        st = (Stats * nb)()
+        _check(lib().rte_solve_fd(self.h, nb, arr, barr, _p(om), C.byref(params), _p(img), _p(refl),
+                                  _p(fld) if field else None, st))
+        img = img.reshape(nb, 2, ny, nx)
+        refl = refl.reshape(nb, 2, ny, nx)
+        out = (img[:, 0] + 1j * img[:, 1], refl[:, 0] + 1j * refl[:, 1], [_stats_dict(s) for s in st])
+        if field:
+            f = fld.reshape(2, K, media[0].N)
+            out = out + (f[0] + 1j * f[1],)
+        return out
 
     def solve(self, shape, bounds, mu_a, mu_s, g, beam, params, n=1.0, c=2.9979e10):
         m = self.medium(shape, bounds, mu_a, mu_s, g, n, c)

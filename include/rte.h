@@ -179,6 +179,26 @@ int rte_solve_images(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
 int rte_solve_field(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte_params *params,
                     double *field, double *exitance, double *reflectance, rte_stats *stats);
 
+/* Frequency-domain solves of intensity-modulated beams: omega[i] (rad/s) is
+ * the reference's [Source] modulation, which turns mu_a into
+ * mu_a + i omega / (c/n) (CPU package/src/RTE_uniform.c:120-177, Inits.c:60).
+ * Source, ballistic term, propagation A (pair factor e^{-tau - i omega n r / c},
+ * self block at the complex mu_t) and the Neumann series are complex; the
+ * scatter and the image functionals are real.  Outputs per solve: exitance
+ * and reflectance (may be NULL) as 2 planes of nx*ny (real, imaginary), i.e.
+ * nb*2*nx*ny doubles; field (may be NULL) the final complex field of solve 0
+ * as 2 planes of K*N (real, imaginary), coefficient-major, reference units.
+ * Stop rules as rte_solve_media: lambda on the real parts of the complex-basis
+ * bottom sums (StopIteration, CPU package/src/Kernels_selfsub.c:1184-1233,
+ * on a complex field), delta on |dE| / |E| of the complex total exitance.
+ * stats report the real parts.  This is the series the reference's modulated
+ * branch sets out to compute; that branch itself accumulates into the
+ * continuous-wave field, normalises the source twice and propagates the
+ * source once (RTE_uniform.c:128-160), which rte_solve_fd does not repeat. */
+int rte_solve_fd(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_beam *beams, const double *omega,
+                 const rte_params *params, double *exitance, double *reflectance, double *field,
+                 rte_stats *stats);
+
 /* ---- derivatives ----------------------------------------------------------
  * Reverse mode through the Neumann series as it ran (same term count): one
  * adjoint series with the transpose of A (reciprocity) and one back-projection

@@ -80,6 +80,14 @@ def lib():
                                          C.POINTER(C.c_int), C.c_int, dp, C.POINTER(Stats)]
         L.orc_bottom_sum.argtypes = [C.POINTER(Grid), C.c_int, dp]
         L.orc_bottom_sum.restype = C.c_double
+        L.orc_self_block_c.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                       C.c_double, C.c_int, dp]
+        L.orc_propagate_c.argtypes = [C.POINTER(Grid), C.POINTER(Optics), C.c_int, C.c_double, C.c_double, dp, dp]
+        L.orc_source_c.argtypes = [C.POINTER(Grid), C.POINTER(Optics), C.POINTER(Beam), C.c_int, C.c_double, dp, dp]
+        L.orc_bottom_sum_c.argtypes = [C.POINTER(Grid), C.c_int, dp]
+        L.orc_bottom_sum_c.restype = C.c_double
+        L.orc_solve_fd.argtypes = [C.POINTER(Grid), C.POINTER(Optics), C.POINTER(Beam), C.POINTER(Params),
+                                   C.c_double, dp, dp, dp, C.POINTER(Stats)]
         _LIB = L
     return _LIB
 
@@ -90,6 +98,10 @@ def _p(a):
 
 def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _c128(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
 
 
 class Problem:
@@ -194,6 +206,49 @@ class Problem:
     def bottom_sum(self, W, L):
         return lib().orc_bottom_sum(C.byref(self.grid), L, _p(_f64(W)))
 
+    # -- frequency domain: complex mu_t = mu_a + mu_s + i kim, kim = omega n / c --
+    def kim(self, omega):
+        return omega * self.optics.n / self.optics.c
+
+    def solve_fd(self, beam, L, omega, stop_rule=STOP_LAMBDA, tol=1e-6, max_terms=1000, prop_thresh=1e30,
+                 exit_quad=QUAD_REF, field=False):
+        """Complex exitance and reflectance of an intensity-modulated beam at
+        angular frequency omega (rad/s)."""
+        nx, ny, _ = self.shape
+        K = (L + 1) ** 2
+        img = np.zeros(nx * ny, dtype=np.complex128)
+        refl = np.zeros(nx * ny, dtype=np.complex128)
+        w = np.zeros(K * self.N, dtype=np.complex128) if field else None
+        st = Stats()
+        prm = Params(L, max_terms, stop_rule, tol, prop_thresh, exit_quad)
+        rc = lib().orc_solve_fd(C.byref(self.grid), C.byref(self.optics), C.byref(Beam(*beam)), C.byref(prm),
+                                self.kim(omega), _p(img), _p(refl), _p(w) if field else None, C.byref(st))
+        if rc != 0:
+            raise RuntimeError("orc_solve_fd failed: %d" % rc)
+        out = dict(exitance=img.reshape(ny, nx), reflectance=refl.reshape(ny, nx), terms=st.terms,
+                   lam=st.lambda_last, rel=st.rel_change_last, ballistic_total=st.ballistic_total)
+        if field:
+            out["w"] = w.reshape(K, self.N)
+        return out
+
+    def source_c(self, beam, L, omega):
+        K = (L + 1) ** 2
+        s0 = np.zeros(K * self.N, dtype=np.complex128)
+        ball = np.zeros(self.shape[0] * self.shape[1], dtype=np.complex128)
+        cols = lib().orc_source_c(C.byref(self.grid), C.byref(self.optics), C.byref(Beam(*beam)), L,
+                                  self.kim(omega), _p(s0), _p(ball))
+        return s0.reshape(K, self.N), ball, cols
+
+    def propagate_c(self, W, L, omega, prop_thresh=1e30):
+        W = _c128(W)
+        out = np.zeros_like(W)
+        lib().orc_propagate_c(C.byref(self.grid), C.byref(self.optics), L, prop_thresh, self.kim(omega), _p(W),
+                              _p(out))
+        return out
+
+    def bottom_sum_c(self, W, L):
+        return lib().orc_bottom_sum_c(C.byref(self.grid), L, _p(_c128(W)))
+
 
 def quad_weights_refl(L, quad=QUAD_REF):
     R = np.zeros(L + 1)
@@ -218,6 +273,14 @@ def self_block(L, dx, dy, dz, mu_t, c_m=1.0, method=0):
     K = (L + 1) ** 2
     S = np.zeros(K * K)
     lib().orc_self_block(L, dx, dy, dz, mu_t, c_m, method, _p(S))
+    return S.reshape(K, K)
+
+
+def self_block_c(L, dx, dy, dz, mu_t, c_m=1.0, method=0):
+    """S(mu_t) for complex mu_t (interleaved in C, complex128 here)."""
+    K = (L + 1) ** 2
+    S = np.zeros(K * K, dtype=np.complex128)
+    lib().orc_self_block_c(L, dx, dy, dz, mu_t.real, mu_t.imag, c_m, method, _p(S))
     return S.reshape(K, K)
 
 

@@ -15,6 +15,7 @@
  */
 #include "rte_oracle.h"
 
+#include <complex.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -793,4 +794,305 @@ int orc_solve_field(const orc_grid *g, const orc_optics *o, const orc_beam *b, c
                     double *exitance, double *w_out, orc_stats *st)
 {
     return solve_impl(g, o, b, p, exitance, w_out, NULL, st);
+}
+
+/* ---------------------------------------------------------------------------
+ * Frequency domain: complex mu_t = mu_a + mu_s + i kim, kim = omega n / c
+ * (CPU package/src/RTE_uniform.c:126).  Fields are complex (interleaved
+ * re, im doubles), still in the real SH basis: A and S become complex, D
+ * stays real.  Pair factor e^{-mu_t r}/(c r^2) with complex mu_t
+ * (Kernels_selfsub.c:1046; e^{-tau - i kim r} for a heterogeneous medium),
+ * self block (1 - e^{-mu beta}) sin(theta) / (c mu dV) (Kernels_selfsub.c:927,
+ * 948), source and ballistic with the complex column attenuation
+ * (Dist.c:1097-1103 evaluated with complex mu_t).
+ * ------------------------------------------------------------------------- */
+
+typedef double complex dcx;
+
+/* S(mu) for complex mu, interleaved K x K; method 0 direct 200x200 quadrature
+ * (the reference's), 1 the moment series with complex coefficients. */
+int orc_self_block_c(int L, double dx, double dy, double dz, double mu_re, double mu_im, double c_m, int method,
+                     double *S_out)
+{
+    int K = ncoef(L), NW = ANG_RES * ANG_RES;
+    double dth = M_PI / ANG_RES, dph = 2.0 * M_PI / ANG_RES;
+    double dV = dx * dy * dz;
+    dcx mu = CMPLX(mu_re, mu_im);
+    dcx *S = (dcx *)S_out;
+    if (method == 0) {
+        double *beta = malloc(sizeof(double) * NW), *sn;
+        orc_self_beta(dx, dy, dz, beta);
+        double *R = self_grid_harmonics(L, &sn);
+        dcx *wt = malloc(sizeof(dcx) * NW);
+        for (int w = 0; w < NW; ++w) wt[w] = (1.0 - cexp(-mu * beta[w])) * sn[w];
+        dcx scale = dth * dph / (c_m * mu * dV);
+#pragma omp parallel for schedule(dynamic)
+        for (int a = 0; a < K; ++a) {
+            for (int b = 0; b < K; ++b) {
+                dcx s = 0.0;
+                for (int w = 0; w < NW; ++w) s += R[(size_t)a * NW + w] * R[(size_t)b * NW + w] * wt[w];
+                S[a * K + b] = s * scale;
+            }
+        }
+        free(R); free(sn); free(beta); free(wt);
+        return 0;
+    }
+    double *M = malloc(sizeof(double) * (size_t)NMOM * K * K);
+    orc_self_moments(L, dx, dy, dz, NMOM, M);
+    double scale = dth * dph / (c_m * dV);
+    for (int ab = 0; ab < K * K; ++ab) S[ab] = 0.0;
+    dcx coef = 1.0;
+    for (int k = 0; k < NMOM; ++k) {
+        coef = (k == 0) ? 1.0 : coef * (-mu) / (double)(k + 1);
+        for (int ab = 0; ab < K * K; ++ab) S[ab] += coef * M[(size_t)k * K * K + ab];
+    }
+    for (int ab = 0; ab < K * K; ++ab) S[ab] *= scale;
+    free(M);
+    return 0;
+}
+
+/* complex A on a complex K*N field (interleaved), every destination */
+int orc_propagate_c(const orc_grid *g, const orc_optics *o, int L, double prop_thresh, double kim,
+                    const double *W_in, double *out_)
+{
+    geo G;
+    geo_init(&G, g);
+    int K = ncoef(L), N = G.N;
+    const dcx *W = (const dcx *)W_in;
+    dcx *out = (dcx *)out_;
+    int NX = 2 * G.nx - 1, NY = 2 * G.ny - 1, NZ = 2 * G.nz - 1;
+    size_t ND = (size_t)NX * NY * NZ;
+    double c_m = o->c / o->n;
+    double *mut = malloc(sizeof(double) * N);
+    for (int v = 0; v < N; ++v) mut[v] = o->mu_a[v] + o->mu_s[v];
+    int uniform = is_uniform(mut, N);
+    double *Ytab = malloc(sizeof(double) * ND * K);
+    double *rtab = malloc(sizeof(double) * ND);
+#pragma omp parallel for
+    for (long q = 0; q < (long)ND; ++q) {
+        int ix = (int)(q % NX), iy = (int)((q / NX) % NY), iz = (int)(q / ((size_t)NX * NY));
+        double ddx = G.dx * (ix - (G.nx - 1)), ddy = G.dy * (iy - (G.ny - 1)), ddz = G.dz * (iz - (G.nz - 1));
+        double r = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+        rtab[q] = r;
+        if (r > 0.0) orc_ylm_real(L, acos(ddz / r), atan2(ddy, ddx), Ytab + q * K);
+        else for (int a = 0; a < K; ++a) Ytab[q * K + a] = 0.0;
+    }
+    dcx *S = malloc(sizeof(dcx) * (size_t)K * K * (uniform ? 1 : N));
+    if (uniform) {
+        orc_self_block_c(L, G.dx, G.dy, G.dz, mut[0], kim, c_m, 0, (double *)S);
+    } else {
+        double *M = malloc(sizeof(double) * (size_t)NMOM * K * K);
+        orc_self_moments(L, G.dx, G.dy, G.dz, NMOM, M);
+        double scale = (M_PI / ANG_RES) * (2.0 * M_PI / ANG_RES) / (c_m * G.dV);
+#pragma omp parallel for
+        for (int v = 0; v < N; ++v) {
+            dcx *Sv = S + (size_t)v * K * K, mu = CMPLX(mut[v], kim), coef = 1.0;
+            for (int ab = 0; ab < K * K; ++ab) Sv[ab] = 0.0;
+            for (int k = 0; k < NMOM; ++k) {
+                coef = (k == 0) ? 1.0 : coef * (-mu) / (double)(k + 1);
+                for (int ab = 0; ab < K * K; ++ab) Sv[ab] += coef * M[(size_t)k * K * K + ab];
+            }
+            for (int ab = 0; ab < K * K; ++ab) Sv[ab] *= scale;
+        }
+        free(M);
+    }
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int d = 0; d < N; ++d) {
+        int di = d % G.nx, dj = (d / G.nx) % G.ny, dk = d / (G.nx * G.ny);
+        dcx acc[64];
+        for (int a = 0; a < K; ++a) acc[a] = 0.0;
+        for (int s = 0; s < N; ++s) {
+            if (s == d) continue;
+            int si = s % G.nx, sj = (s / G.nx) % G.ny, sk = s / (G.nx * G.ny);
+            size_t q = (size_t)(di - si + G.nx - 1) +
+                       (size_t)NX * ((size_t)(dj - sj + G.ny - 1) + (size_t)NY * (dk - sk + G.nz - 1));
+            double r = rtab[q];
+            if (!(r < prop_thresh)) continue;
+            double tau = uniform ? mut[0] * r : tau_pair(&G, mut, di, dj, dk, si, sj, sk, r);
+            dcx kf = cexp(CMPLX(-tau, -kim * r)) / (c_m * r * r);
+            const double *y = Ytab + q * K;
+            dcx p = 0.0;
+            for (int b = 0; b < K; ++b) p += y[b] * W[(size_t)b * N + s];
+            p *= kf;
+            for (int a = 0; a < K; ++a) acc[a] += p * y[a];
+        }
+        const dcx *Sd = S + (uniform ? 0 : (size_t)d * K * K);
+        for (int a = 0; a < K; ++a) {
+            dcx s = 0.0;
+            for (int b = 0; b < K; ++b) s += Sd[a * K + b] * W[(size_t)b * N + d];
+            out[(size_t)a * N + d] = (acc[a] + s) * G.dV;
+        }
+    }
+    free(Ytab); free(rtab); free(S); free(mut);
+    geo_free(&G);
+    return 0;
+}
+
+/* complex first-scatter source and ballistic image (interleaved); returns the
+ * number of beam columns (0 = error) */
+int orc_source_c(const orc_grid *g, const orc_optics *o, const orc_beam *b, int L, double kim, double *s0_,
+                 double *ball_)
+{
+    geo G;
+    geo_init(&G, g);
+    int K = ncoef(L), N = G.N;
+    dcx *s0 = (dcx *)s0_, *ball = (dcx *)ball_;
+    double diam = b->diam + 1.0e-16;
+    for (size_t i = 0; i < (size_t)K * N; ++i) s0[i] = 0.0;
+    int cols = 0;
+    for (int j = 0; j < G.ny; ++j)
+        for (int i = 0; i < G.nx; ++i) {
+            double rad = sqrt((G.xs[i] - b->pos_x) * (G.xs[i] - b->pos_x) + (G.ys[j] - b->pos_y) * (G.ys[j] - b->pos_y));
+            if (rad <= diam / 2.0) ++cols;
+        }
+    if (cols == 0) { geo_free(&G); return 0; }
+    double fact = b->mag / (double)cols;
+    for (int j = 0; j < G.ny; ++j)
+        for (int i = 0; i < G.nx; ++i) {
+            double rad = sqrt((G.xs[i] - b->pos_x) * (G.xs[i] - b->pos_x) + (G.ys[j] - b->pos_y) * (G.ys[j] - b->pos_y));
+            if (ball) ball[i + G.nx * j] = 0.0;
+            if (!(rad <= diam / 2.0)) continue;
+            dcx T = 1.0;
+            for (int k = 0; k < G.nz; ++k) {
+                int v = i + G.nx * (j + G.ny * k);
+                dcx mt = CMPLX(o->mu_a[v] + o->mu_s[v], kim);
+                dcx att = cexp(-mt * G.dz);
+                dcx base = fact * (o->mu_s[v] / mt) * T * (1.0 - att) / G.dV;
+                double gl = 1.0;
+                for (int l = 0; l <= L; ++l) {
+                    s0[(size_t)(l * l + l) * N + v] = gl * sqrt((2.0 * l + 1.0) / (4.0 * M_PI)) * base;
+                    gl *= o->g;
+                }
+                T *= att;
+            }
+            if (ball) ball[i + G.nx * j] = fact * T;
+        }
+    geo_free(&G);
+    return cols;
+}
+
+/* Sum over the bottom layer of Re of every complex-basis coefficient of a
+ * complex field given in the real basis (Kernels_selfsub.c:1194-1206): with
+ * c_lm = (r_lm - i r_l,-m)/sqrt2 and c_l,-m = (-1)^m (r_lm + i r_l,-m)/sqrt2,
+ * Re c_lm + Re c_l,-m = sqrt2 Re r_lm for even m and sqrt2 Im r_l,-m for odd m. */
+double orc_bottom_sum_c(const orc_grid *g, int L, const double *W_)
+{
+    const dcx *W = (const dcx *)W_;
+    int N = g->nx * g->ny * g->nz, base = g->nx * g->ny * (g->nz - 1);
+    double s = 0.0;
+    for (int l = 0; l <= L; ++l)
+        for (int m = -l; m <= l; ++m) {
+            double wgt;
+            int use_im;
+            if (m == 0) { wgt = 1.0; use_im = 0; }
+            else if (m > 0 && (m % 2) == 0) { wgt = M_SQRT2; use_im = 0; }
+            else if (m < 0 && (-m % 2) == 1) { wgt = M_SQRT2; use_im = 1; }
+            else continue;
+            const dcx *row = W + (size_t)(l * l + l + m) * N + base;
+            double t = 0.0;
+            for (int v = 0; v < g->nx * g->ny; ++v) t += use_im ? cimag(row[v]) : creal(row[v]);
+            s += wgt * t;
+        }
+    return s;
+}
+
+/* bottom exitance and top reflectance of a complex field (no ballistic) */
+static void images_c(const orc_grid *g, const orc_optics *o, int L, int quad, const dcx *w, dcx *ex, dcx *rf)
+{
+    int K = ncoef(L), N = g->nx * g->ny * g->nz, P = g->nx * g->ny;
+    double *part = malloc(sizeof(double) * (size_t)K * N), *img = malloc(sizeof(double) * P);
+    for (int h = 0; h < 2; ++h) {
+        for (size_t i = 0; i < (size_t)K * N; ++i) part[i] = h ? cimag(w[i]) : creal(w[i]);
+        orc_exitance(g, o, L, quad, part, img);
+        for (int i = 0; i < P; ++i) ex[i] = h ? ex[i] + CMPLX(0.0, img[i]) : img[i];
+        if (rf) {
+            orc_reflectance(g, o, L, quad, part, img);
+            for (int i = 0; i < P; ++i) rf[i] = h ? rf[i] + CMPLX(0.0, img[i]) : img[i];
+        }
+    }
+    free(part);
+    free(img);
+}
+
+/* Frequency-domain solve: the Neumann series of the complex operators,
+ * exitance = ballistic + bottom exitance of A sum_q (D A)^q s0 (complex, nx*ny
+ * interleaved); refl (or NULL) the top reflectance; w_out (or NULL) the final
+ * complex field.  Stop rules as orc_solve: lambda on the real parts of the
+ * complex-basis bottom sums, delta on |dE| / |E| of the complex total. */
+int orc_solve_fd(const orc_grid *g, const orc_optics *o, const orc_beam *b, const orc_params *p, double kim,
+                 double *exitance_, double *refl_, double *w_out_, orc_stats *st)
+{
+    int L = p->L, K = ncoef(L), N = g->nx * g->ny * g->nz, P = g->nx * g->ny;
+    size_t KN = (size_t)K * N;
+    dcx *exitance = (dcx *)exitance_, *refl = (dcx *)refl_, *w_out = (dcx *)w_out_;
+    dcx *s0 = malloc(sizeof(dcx) * KN), *ball = malloc(sizeof(dcx) * P);
+    if (!orc_source_c(g, o, b, L, kim, (double *)s0, (double *)ball)) { free(s0); free(ball); return -1; }
+    dcx *W = malloc(sizeof(dcx) * KN), *AW = malloc(sizeof(dcx) * KN), *out = malloc(sizeof(dcx) * KN);
+    dcx *img = malloc(sizeof(dcx) * P), *rimg = malloc(sizeof(dcx) * P);
+    double *mus = (double *)o->mu_s;
+    dcx btot = 0.0;
+    for (int i = 0; i < P; ++i) btot += ball[i];
+    memcpy(W, s0, sizeof(dcx) * KN);
+    memcpy(out, s0, sizeof(dcx) * KN);
+    int Lloop = (o->g != 0.0) ? L : 0;
+    int n, max_terms = p->max_terms > 0 ? p->max_terms : 1000;
+    double lam = 0.0, rel = 0.0, c_m = o->c / o->n;
+    /* D: c_m g^l mu_s, real */
+#define SCATTER_C(F)                                                                 \
+    for (int l = 0; l <= L; ++l) {                                                   \
+        double gl = pow(o->g, (double)l);                                            \
+        for (int m = -l; m <= l; ++m) {                                              \
+            dcx *row = (F) + (size_t)(l * l + l + m) * N;                            \
+            for (int v = 0; v < N; ++v) row[v] = c_m * gl * mus[v] * row[v];         \
+        }                                                                            \
+    }
+    if (p->stop_rule == ORC_STOP_LAMBDA) {
+        double fi = 1.0;
+        n = 1;
+        while (!stop_lambda(&fi, n, orc_bottom_sum_c(g, L, (double *)W), orc_bottom_sum_c(g, L, (double *)out),
+                            &lam)) {
+            if (n >= max_terms) break;
+            orc_propagate_c(g, o, Lloop, p->prop_thresh, kim, (double *)W, (double *)AW);
+            if (Lloop < L)
+                for (size_t i = (size_t)ncoef(Lloop) * N; i < KN; ++i) AW[i] = 0.0;
+            SCATTER_C(AW)
+            memcpy(W, AW, sizeof(dcx) * KN);
+            for (size_t i = 0; i < KN; ++i) out[i] += W[i];
+            ++n;
+        }
+        orc_propagate_c(g, o, L, p->prop_thresh, kim, (double *)out, (double *)AW);
+        images_c(g, o, L, p->exit_quad, AW, img, refl ? rimg : NULL);
+        for (int i = 0; i < P; ++i) exitance[i] = ball[i] + img[i];
+        if (refl) memcpy(refl, rimg, sizeof(dcx) * P);
+        if (w_out) memcpy(w_out, AW, sizeof(dcx) * KN);
+    } else {
+        dcx T = btot;
+        for (int i = 0; i < P; ++i) exitance[i] = ball[i];
+        if (w_out) for (size_t i = 0; i < KN; ++i) w_out[i] = 0.0;
+        if (refl) for (int i = 0; i < P; ++i) refl[i] = 0.0;
+        n = 0;
+        for (;;) {
+            orc_propagate_c(g, o, L, p->prop_thresh, kim, (double *)W, (double *)AW);
+            images_c(g, o, L, p->exit_quad, AW, img, refl ? rimg : NULL);
+            dcx dT = 0.0;
+            for (int i = 0; i < P; ++i) { exitance[i] += img[i]; dT += img[i]; }
+            if (w_out) for (size_t i = 0; i < KN; ++i) w_out[i] += AW[i];
+            if (refl) for (int i = 0; i < P; ++i) refl[i] += rimg[i];
+            T += dT;
+            ++n;
+            rel = (cabs(T) != 0.0) ? cabs(dT) / cabs(T) : 0.0;
+            if (rel < p->tol || n >= max_terms) break;
+            SCATTER_C(AW)
+            memcpy(W, AW, sizeof(dcx) * KN);
+        }
+    }
+#undef SCATTER_C
+    if (st) {
+        st->terms = n;
+        st->lambda_last = lam;
+        st->rel_change_last = rel;
+        st->ballistic_total = creal(btot);
+    }
+    free(s0); free(ball); free(W); free(AW); free(out); free(img); free(rimg);
+    return 0;
 }

@@ -117,6 +117,18 @@ int orc_solve_field(const orc_grid *g, const orc_optics *o, const orc_beam *b,
 int orc_ref_derivative(const orc_grid *g, const orc_optics *o, const orc_beam *b, const orc_params *p,
                        const int *region, int which, double *dimg, orc_stats *st);
 
+/* Frequency domain (complex mu_t = mu_a + mu_s + i kim, kim = omega n / c).
+ * Complex arrays are interleaved (re, im) doubles, fields coefficient-major. */
+int orc_self_block_c(int L, double dx, double dy, double dz, double mu_re, double mu_im, double c_m,
+                     int method, double *S);
+int orc_propagate_c(const orc_grid *g, const orc_optics *o, int L, double prop_thresh, double kim,
+                    const double *W, double *out);
+int orc_source_c(const orc_grid *g, const orc_optics *o, const orc_beam *b, int L, double kim,
+                 double *s0, double *ballistic);
+double orc_bottom_sum_c(const orc_grid *g, int L, const double *W);
+int orc_solve_fd(const orc_grid *g, const orc_optics *o, const orc_beam *b, const orc_params *p, double kim,
+                 double *exitance, double *reflectance, double *w_out, orc_stats *st);
+
 /* bottom-layer sums of the stop rule (complex-basis real parts) */
 double orc_bottom_sum(const orc_grid *g, int L, const double *W);
 

@@ -25,3 +25,10 @@ def golden_ref():
     import json
     with open(os.path.join(ROOT, "tests", "golden", "ref_cpu_uniform.json")) as f:
         return json.load(f)["cases"]
+
+
+@pytest.fixture(scope="session")
+def golden_mod():
+    import json
+    with open(os.path.join(ROOT, "tests", "golden", "ref_cpu_modulated.json")) as f:
+        return json.load(f)["cases"]

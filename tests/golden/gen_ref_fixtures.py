@@ -1,6 +1,8 @@
-"""Writes tests/golden/ref_cpu_uniform.json: exitance images and Neumann term
-counts produced by the reference CPU package (uniform media, FP64), built from
-its own sources by oracle/Makefile.ref into oracle/_ref/rte_cpu.
+"""Writes tests/golden/ref_cpu_uniform.json (exitance images and Neumann term
+counts produced by the reference CPU package, uniform media, FP64) and
+tests/golden/ref_cpu_modulated.json (the same binary with a modulated source:
+transmitted, reflected and phase images, every printed term count), built
+from its own sources by oracle/Makefile.ref into oracle/_ref/rte_cpu.
 
 Run in the build container only (it needs /root/reference to build the
 binary):  make -f oracle/Makefile.ref && python tests/golden/gen_ref_fixtures.py
@@ -32,6 +34,14 @@ CASES = [
      (0.1, 0.05, 0.15), 10.0),
     ("slab_7x7x3_L4_g0.9_wide_beam", (7, 7, 3), (-0.35, 0.35, -0.35, 0.35, 0.0, 0.3), 0.1, 3.0, 0.9, 1.0, 4,
      (0.0, 0.0, 0.35), 10.0),
+]
+
+# modulated sources (CPU package/src/RTE_uniform.c:120-177): name, case of CASES it
+# modifies, modulation (angular frequency, rad/s)
+MOD_CASES = [
+    ("config1_L3_g0.8_mod6.3e9", "config1_L3_g0.8", 6.283185307179586e9),
+    ("config1_L1_g0_mod3e10", "config1_L1_g0", 3.0e10),
+    ("box_6x5x4_L2_offcentre_n1.37_mod2e10", "box_6x5x4_L2_offcentre_n1.37", 2.0e10),
 ]
 
 INI = """[Runtime]
@@ -69,18 +79,19 @@ posX = {beam[0]!r}
 posY = {beam[1]!r}
 diam = {beam[2]!r}
 magnitude = 1.0
-modulation = 0.0
+modulation = {mod!r}
 """
 
 
-def run_case(case):
+def run_case(case, mod=0.0):
     name, (nx, ny, nz), b, mua, mus, g, n, L, beam, pt = case
     with tempfile.TemporaryDirectory() as d:
         with open(os.path.join(d, "in.ini"), "w") as f:
-            f.write(INI.format(L=L, nx=nx, ny=ny, nz=nz, b=b, pt=pt, mua=mua, mus=mus, g=g, n=n, beam=beam))
-        out = subprocess.run([BIN, "in.ini"], cwd=d, capture_output=True, text=True, check=True,
+            f.write(INI.format(L=L, nx=nx, ny=ny, nz=nz, b=b, pt=pt, mua=mua, mus=mus, g=g, n=n, beam=beam,
+                               mod=mod))
+        out = subprocess.run([BIN, "in.ini"], cwd=d, capture_output=True, text=True, check=True, timeout=600,
                              env=dict(os.environ, OMP_NUM_THREADS="8")).stdout
-        terms = int(re.findall(r"Neumann series: (\d+)", out)[-1])
+        counts = [int(t) for t in re.findall(r"Neumann series: (\d+)", out)]
         photons = float(re.findall(r"Number of photons total: ([-+0-9.eE]+)", out)[-1])
         images = {}
         for key, fname in (("trans_image", "trans.out"), ("refl_image", "refl.out")):
@@ -88,13 +99,35 @@ def run_case(case):
             rx, ry = struct.unpack("ii", raw[:8])
             vals = np.frombuffer(raw[8:], dtype=np.float64)
             images[key] = vals[: rx * ry].reshape(ry, rx).tolist()
-    return dict(name=name, shape=[nx, ny, nz], bounds=list(b), mua=mua, mus=mus, g=g, n=n, L=L,
-                beam=dict(pos_x=beam[0], pos_y=beam[1], diam=beam[2], mag=1.0), prop_thresh=pt,
-                terms=terms, photons_printed=photons, **images)
+            if mod:
+                images[key + "_imag"] = vals[rx * ry: 2 * rx * ry].reshape(ry, rx).tolist()
+        if mod:
+            raw = open(os.path.join(d, "phase.out"), "rb").read()
+            rx, ry = struct.unpack("ii", raw[:8])
+            images["phase_image"] = np.frombuffer(raw[8:], dtype=np.float64)[: rx * ry].reshape(ry, rx).tolist()
+    res = dict(name=name, shape=[nx, ny, nz], bounds=list(b), mua=mua, mus=mus, g=g, n=n, L=L,
+               beam=dict(pos_x=beam[0], pos_y=beam[1], diam=beam[2], mag=1.0), prop_thresh=pt,
+               terms=counts[-1], photons_printed=photons, **images)
+    if mod:
+        res.update(modulation=mod, terms_printed=counts)
+    return res
 
 
 def main():
     res = [run_case(c) for c in CASES]
+    byname = {c[0]: c for c in CASES}
+    mod = []
+    for name, base, om in MOD_CASES:
+        r = run_case((name,) + tuple(byname[base][1:]), mod=om)
+        mod.append(r)
+        print(r["name"], r["terms_printed"])
+    with open(os.path.join(HERE, "ref_cpu_modulated.json"), "w") as f:
+        json.dump(dict(source="reference CPU package (CPU package/src), built by oracle/Makefile.ref, "
+                              "[Source] modulation != 0",
+                       image_layout="[j][i] (OutputImage order); trans/refl _imag: the imaginary planes; "
+                                    "phase_image: PhaseFile (degrees); terms_printed: every 'Neumann series' "
+                                    "line (CW solve, modulated solve, final)",
+                       cases=mod), f, indent=1)
     with open(os.path.join(HERE, "ref_cpu_uniform.json"), "w") as f:
         json.dump(dict(source="reference CPU package (CPU package/src), built by oracle/Makefile.ref",
                        image_layout="trans_image / refl_image [j][i], j = y index, i = x index (OutputImage "

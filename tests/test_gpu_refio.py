@@ -2,7 +2,8 @@
 the transmitted image written in the reference's OutputImage format, and the
 reflectance image, equal the reference CPU package's for the same file (FP64
 within 1e-9, same lambda-rule term count); the command-line entry with a fixed
-term count writes both files."""
+term count writes both files; a modulated source adds the frequency-domain
+solve to the files."""
 import numpy as np
 import pytest
 
@@ -46,6 +47,44 @@ def test_parameter_file_end_to_end(ctx, golden_ref, tmp_path):
         os.chdir(cwd)
     assert rel_max(t, np.array(case["trans_image"])) < 1e-4
     assert rel_max(r, np.array(case["refl_image"])) < 1e-4
+
+
+def test_modulated_parameter_file(ctx, golden_ref, golden_mod, tmp_path):
+    """A parameter file with a [Source] modulation: PhaseFile is the phase of
+    the frequency-domain exitance (ComputePhaseImage's quadrature), TransFile
+    and ReflectFile are CombineOuts' half continuous-wave plus half the real part
+    of the modulated solve, all against the oracle's series (FP64 path, 1e-9).
+    Without a modulation PhaseFile holds the phase of the real exitance, 0."""
+    import os
+    cwd = os.getcwd()
+    try:
+        os.chdir(str(tmp_path))
+        for case in golden_mod:
+            print("modulated file", case["name"], flush=True)
+            (tmp_path / "m.ini").write_text(ini_for(case, mod=case["modulation"]))
+            img, st, refl = run.solve_file("m.ini", precision="fp64", ctx=ctx, quiet=True)
+            t, ti = refio.read_image("trans.out")
+            r, ri = refio.read_image("refl.out")
+            ph = refio.read_phase_image("phase.out")
+            b = case["beam"]
+            beam = (b["pos_x"], b["pos_y"], b["diam"], 1.0)
+            P = O.Problem(tuple(case["shape"]), tuple(case["bounds"]), case["mua"], case["mus"], case["g"],
+                          n=case["n"])
+            dc = P.solve(beam, case["L"], prop_thresh=case["prop_thresh"], reflectance=True)
+            ac = P.solve_fd(beam, case["L"], case["modulation"], prop_thresh=case["prop_thresh"])
+            assert st["terms"] == dc["terms"] and st["terms_fd"] == ac["terms"]
+            assert rel_max(t, 0.5 * dc["exitance"] + 0.5 * ac["exitance"].real) < 1e-9
+            assert rel_max(r, 0.5 * dc["reflectance"] + 0.5 * ac["reflectance"].real) < 1e-9
+            assert not np.any(ti) and not np.any(ri)
+            prob = refio.load("m.ini")
+            want = run.phase_image(ac["exitance"], run.ballistic_image(prob, case["modulation"] * case["n"] / 2.9979e10))
+            assert np.max(np.abs(ph - want)) < 1e-6
+            assert np.min(ph) < -1.0                      # degrees of delay
+        (tmp_path / "c.ini").write_text(ini_for(golden_ref[0]))
+        run.solve_file("c.ini", precision="fp64", ctx=ctx, quiet=True)
+        assert not np.any(refio.read_phase_image("phase.out"))
+    finally:
+        os.chdir(cwd)
 
 
 def test_final_field_against_oracle(ctx):

@@ -15,6 +15,7 @@ from tests.helpers import rel_max
 INI = """[Runtime]
 TransFile = trans.out
 ReflectFile = refl.out
+PhaseFile = phase.out
 
 [Algorithm]
 nL = {L}
@@ -44,15 +45,15 @@ posX = {beam[0]!r}
 posY = {beam[1]!r}
 diam = {beam[2]!r}
 magnitude = 1.0
-modulation = 0.0
+modulation = {mod!r}
 """
 
 
-def ini_for(case):
+def ini_for(case, mod=0.0):
     nx, ny, nz = case["shape"]
     b = case["beam"]
     return INI.format(L=case["L"], nx=nx, ny=ny, nz=nz, b=case["bounds"], pt=case["prop_thresh"], mua=case["mua"],
-                      mus=case["mus"], g=case["g"], n=case["n"], beam=(b["pos_x"], b["pos_y"], b["diam"]))
+                      mus=case["mus"], g=case["g"], n=case["n"], beam=(b["pos_x"], b["pos_y"], b["diam"]), mod=mod)
 
 
 def test_ini_rules():
