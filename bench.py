@@ -10,6 +10,7 @@ in HBM before the timed region; every step rebuilds their path tables.
     python bench.py [--gpus 1] [--steps 2] [--warmup 1]
     python bench.py --serve [--requests 2000] [--rate 200]   # serving stream (config 4)
     python bench.py --config3                                # gradient + Fisher at 24^3
+    python bench.py --fd [--omega 6.28e8]                    # config 5 with modulated beams
 
 Prints one JSON line.  One GPU only: --gpus N > 1 exits with a message.
 """
@@ -157,6 +158,96 @@ def serve(args):
     ctx.close()
 
 
+def cpu_baseline_fd(shape, bounds, mu_a, mu_s, g, L, omega, terms, budget_s=12.0):
+    """cpu_baseline for the frequency-domain workload: the oracle's complex
+    propagation on a sample of destination voxels, scaled as cpu_baseline."""
+    from oracle import oracle as O
+    N = int(np.prod(shape))
+    P = O.Problem(shape, bounds, mu_a, mu_s, g)
+    rng = np.random.default_rng(0)
+    W = rng.standard_normal(((L + 1) ** 2, N)) + 1j * rng.standard_normal(((L + 1) ** 2, N))
+    t0 = time.perf_counter()
+    P.propagate_c_range(W, L, omega, 0, 0)
+    setup = time.perf_counter() - t0
+    rows, done, el = 128, 0, 0.0
+    while el < budget_s and done < N:
+        n = min(rows, N - done)
+        t0 = time.perf_counter()
+        P.propagate_c_range(W, L, omega, done, done + n)
+        el += time.perf_counter() - t0 - setup
+        done += n
+        rows *= 2
+    per_prop = el * N / done
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=1.0 / (per_prop * terms), unit="solves/s", cores=cores, kind="port",
+                sample="FP64 oracle complex propagation of one 32^3 L=5 medium over %d of %d destination voxels "
+                       "(%.1f s), scaled to a full propagation (%.1f s) x %.1f Neumann terms per solve"
+                       % (done, N, el, per_prop, terms))
+
+
+def frequency_domain(args):
+    """The config-5 batch with every beam intensity-modulated (omega rad/s):
+    complex Neumann series through the complex propagation group."""
+    from doi_amd import media, rte
+    n, L, g = args.n, args.L, 0.9
+    shape, bounds = (n, n, n), media.cube_bounds(n)
+    maps = media.config5_media(args.media, seed=5, n=n)
+    beams5 = media.config5_beams(seed=5, n=n, per_medium=args.beams)
+    ctx = rte.Context(0)
+    mm = [ctx.medium(shape, bounds, ma, ms, g) for ma, ms in maps]
+    med_list = [m for m in mm for _ in range(args.beams)]
+    beam_list = [b for _ in mm for b in beams5]
+    nsolve = len(med_list)
+    prm = rte.make_params(L, precision=rte.PREC_MIXED, stop_rule=rte.STOP_DELTA, tol=1e-6, max_terms=30)
+    for i in range(args.warmup):
+        log("fd warmup step", i)
+        ctx.solve_fd(med_list, beam_list, [args.omega] * nsolve, prm)
+    ctx.profile(True)
+    times, lat = [], []
+    for i in range(args.steps):
+        for m in mm:
+            m.reset_cache()
+        t0 = time.perf_counter()
+        imgs, _, stats = ctx.solve_fd(med_list, beam_list, [args.omega] * nsolve, prm)
+        times.append(time.perf_counter() - t0)
+        lat += [s["latency_ms"] for s in stats]
+        log("fd step %d: %.3f s, terms %s" % (i, times[-1], sorted(set(s["terms"] for s in stats))))
+    prof = ctx.get_profile()
+    ctx.profile(False)
+    ms_per_step = 1000.0 * float(np.mean(times))
+    terms_mean = float(np.mean([s["terms"] for s in stats]))
+    launches = max(prof["prop_launches"], 1)
+    avg_ms = prof["prop_ms"] / launches
+    achieved = prof["prop_flops"] / launches / (avg_ms * 1e-3) / 1e12
+    p64 = rte.make_params(L, precision=rte.PREC_FP64, stop_rule=rte.STOP_DELTA, tol=1e-6, max_terms=30)
+    ref, _, _ = ctx.solve_fd([mm[0]] * args.beams, beams5, [args.omega] * args.beams, p64)
+    err = max(float(np.max(np.abs(imgs[b] - ref[b])) / np.max(np.abs(ref[b]))) for b in range(args.beams))
+    # phase where the exitance is well resolved (>= 1 % of its peak); the corner
+    # pixels of an L = 5 expansion at g = 0.9 can go slightly negative
+    big = np.abs(ref[0]) >= 1e-2 * np.max(np.abs(ref[0]))
+    ph = np.angle(ref[0][big])
+    res = {"metric": "frequency-domain RTE solves/s at 32^3 L=5 (modulated beams), <=1e-4 rel err vs FP64",
+           "value": nsolve / (ms_per_step / 1000.0), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "fp32 complex (fp64 running sums)",
+           "data": "synthetic (seeded lognormal media, config 5)",
+           "config": {"workload": "config5 media x beams, %d^3, L=%d, every beam modulated at omega = %.4g rad/s"
+                                  % (n, L, args.omega), "solves_per_step": nsolve},
+           "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+           "terms_mean": terms_mean, "rel_err_vs_fp64": err,
+           "phase_range_deg_medium0_beam0_resolved_pixels": [float(np.degrees(ph.min())), float(np.degrees(ph.max()))],
+           "roofline": {"bound": "mfma", "pipe": "valu-fp32", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                        "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                        "kernel": "k_prop complex group (cached path)", "avg_launch_ms": avg_ms}}
+    if not args.no_cpu:
+        log("cpu baseline (complex)")
+        res["cpu_baseline"] = cpu_baseline_fd(shape, bounds, maps[0][0], maps[0][1], g, L, args.omega, terms_mean)
+    print(json.dumps(res), flush=True)
+    for m in mm:
+        m.close()
+    ctx.close()
+
+
 def config3(args):
     """Differentiation workload: 24^3 heterogeneous medium, L=5, mixed
     precision; the loss gradient with respect to every voxel's mu_a and mu_s
@@ -219,6 +310,8 @@ def main():
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--plain-sample", type=int, default=2)
     ap.add_argument("--config3", action="store_true", help="time the gradient and Fisher workload instead")
+    ap.add_argument("--fd", action="store_true", help="time the frequency-domain (modulated beam) workload")
+    ap.add_argument("--omega", type=float, default=2.0 * np.pi * 1.0e8, help="modulation, rad/s (--fd)")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="only run 3 plain device copies of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -242,6 +335,8 @@ def main():
         return serve(args)
     if args.config3:
         return config3(args)
+    if args.fd:
+        return frequency_domain(args)
     from doi_amd import media, rte
 
     n, L = args.n, args.L

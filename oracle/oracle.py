@@ -83,6 +83,8 @@ def lib():
         L.orc_self_block_c.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                        C.c_double, C.c_int, dp]
         L.orc_propagate_c.argtypes = [C.POINTER(Grid), C.POINTER(Optics), C.c_int, C.c_double, C.c_double, dp, dp]
+        L.orc_propagate_c_range.argtypes = [C.POINTER(Grid), C.POINTER(Optics), C.c_int, C.c_double, C.c_double, dp,
+                                            dp, C.c_int, C.c_int]
         L.orc_source_c.argtypes = [C.POINTER(Grid), C.POINTER(Optics), C.POINTER(Beam), C.c_int, C.c_double, dp, dp]
         L.orc_bottom_sum_c.argtypes = [C.POINTER(Grid), C.c_int, dp]
         L.orc_bottom_sum_c.restype = C.c_double
@@ -244,6 +246,13 @@ class Problem:
         out = np.zeros_like(W)
         lib().orc_propagate_c(C.byref(self.grid), C.byref(self.optics), L, prop_thresh, self.kim(omega), _p(W),
                               _p(out))
+        return out
+
+    def propagate_c_range(self, W, L, omega, d_begin, d_end, prop_thresh=1e30):
+        W = _c128(W)
+        out = np.zeros_like(W)
+        lib().orc_propagate_c_range(C.byref(self.grid), C.byref(self.optics), L, prop_thresh, self.kim(omega), _p(W),
+                                    _p(out), d_begin, d_end)
         return out
 
     def bottom_sum_c(self, W, L):

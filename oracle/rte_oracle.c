@@ -855,6 +855,13 @@ int orc_self_block_c(int L, double dx, double dy, double dz, double mu_re, doubl
 int orc_propagate_c(const orc_grid *g, const orc_optics *o, int L, double prop_thresh, double kim,
                     const double *W_in, double *out_)
 {
+    return orc_propagate_c_range(g, o, L, prop_thresh, kim, W_in, out_, 0, g->nx * g->ny * g->nz);
+}
+
+/* the same restricted to destinations d_begin <= d < d_end (other rows untouched) */
+int orc_propagate_c_range(const orc_grid *g, const orc_optics *o, int L, double prop_thresh, double kim,
+                          const double *W_in, double *out_, int d_begin, int d_end)
+{
     geo G;
     geo_init(&G, g);
     int K = ncoef(L), N = G.N;
@@ -897,7 +904,7 @@ int orc_propagate_c(const orc_grid *g, const orc_optics *o, int L, double prop_t
         free(M);
     }
 #pragma omp parallel for schedule(dynamic, 4)
-    for (int d = 0; d < N; ++d) {
+    for (int d = d_begin; d < d_end; ++d) {
         int di = d % G.nx, dj = (d / G.nx) % G.ny, dk = d / (G.nx * G.ny);
         dcx acc[64];
         for (int a = 0; a < K; ++a) acc[a] = 0.0;

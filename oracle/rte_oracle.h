@@ -123,6 +123,8 @@ int orc_self_block_c(int L, double dx, double dy, double dz, double mu_re, doubl
                      int method, double *S);
 int orc_propagate_c(const orc_grid *g, const orc_optics *o, int L, double prop_thresh, double kim,
                     const double *W, double *out);
+int orc_propagate_c_range(const orc_grid *g, const orc_optics *o, int L, double prop_thresh, double kim,
+                          const double *W, double *out, int d_begin, int d_end);
 int orc_source_c(const orc_grid *g, const orc_optics *o, const orc_beam *b, int L, double kim,
                  double *s0, double *ballistic);
 double orc_bottom_sum_c(const orc_grid *g, int L, const double *W);
