@@ -101,33 +101,39 @@ __global__ __launch_bounds__(kWave) void k_prop(PropArgs<T> A, const SolveDev<T,
         const int snext = (s + 1 < s1) ? s + 1 : s;
         if constexpr (CPLX) {
             // the real and imaginary halves of one complex field share the pair:
-            // t = kf e^{-i kim r} (p0 + i p1), both projections taken first
-            T wn[K];
-#pragma unroll
-            for (int a = 0; a < K; ++a) wn[a] = Wsrc[1][(size_t)s * K + a];
+            // t = kf e^{-i kim r} (p0 + i p1), both projections taken first.
+            // One set of scalar registers holds W (see below): the imaginary
+            // half is loaded after the real projection, the next source's real
+            // half after the imaginary one, covered by the two axpys.
             const T p0 = pair_dot<T, L>(w, G);
 #pragma unroll
-            for (int a = 0; a < K; ++a) w[a] = wn[a];
-#pragma unroll
-            for (int a = 0; a < K; ++a) wn[a] = Wsrc[0][(size_t)snext * K + a];
-            const T p1 = pair_dot<T, L>(w, G);
-#pragma unroll
-            for (int a = 0; a < K; ++a) w[a] = wn[a];
+            for (int a = 0; a < K; ++a) w[a] = Wsrc[1][(size_t)s * K + a];
             T sn, cs;
             dev_sincos<T>(kim * (r2 * rinv), &sn, &cs);
             const T kc = kf * cs, ks = kf * sn;
+            __builtin_amdgcn_sched_barrier(0);
+            const T p1 = pair_dot<T, L>(w, G);
+#pragma unroll
+            for (int a = 0; a < K; ++a) w[a] = Wsrc[0][(size_t)snext * K + a];
+            __builtin_amdgcn_sched_barrier(0);
             pair_axpy<T, L>(acc[0], kc * p0 + ks * p1, G);
             pair_axpy<T, L>(acc[1], kc * p1 - ks * p0, G);
+            __builtin_amdgcn_sched_barrier(0);
         } else {
+            // One set of scalar registers holds W: the projection reads it, the
+            // next right-hand side (or source) is then loaded into the same
+            // registers, and the axpy, which needs no W, covers the load.  The
+            // scheduling barriers keep the next projection from being hoisted
+            // above the axpy (two live sets of W do not fit the SGPR file).
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
-                T wn[K];
+                const T t = kf * pair_dot<T, L>(w, G);
 #pragma unroll
                 for (int a = 0; a < K; ++a)
-                    wn[a] = (r + 1 < NR) ? Wsrc[r + 1][(size_t)s * K + a] : Wsrc[0][(size_t)snext * K + a];
-                pair_apply<T, L>(acc[r], w, kf, G);
-#pragma unroll
-                for (int a = 0; a < K; ++a) w[a] = wn[a];
+                    w[a] = (r + 1 < NR) ? Wsrc[r + 1][(size_t)s * K + a] : Wsrc[0][(size_t)snext * K + a];
+                __builtin_amdgcn_sched_barrier(0);
+                pair_axpy<T, L>(acc[r], t, G);
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
         if (++si == g.nx) {
