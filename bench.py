@@ -29,8 +29,9 @@ PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 (vector/matrix dense) published peak
 PEAK_HBM_GBS = 8000.0
 # HBM bytes per solve and term of k_prop at 32^3 L=5 (mixed, cached), from rocprofv3
 # FETCH_SIZE + WRITE_SIZE calibrated against a device copy
-# (profiles/k_prop_32cube_L5_4media_hbm_traffic.md): a beam in a group of two, a single beam
-TRAFFIC_PER_SOLVE_TERM = {2: 2.41e9, 1: 4.30e9}
+# (profiles/k_prop_32cube_L5_4media_hbm_traffic.md): a beam in a group of three, of two, alone
+TRAFFIC_PER_SOLVE_TERM = {3: 1.53e9, 2: 2.21e9, 1: 4.45e9}
+DEFAULT_GROUP = 3          # the library's default number of beams per propagation group
 
 
 def log(*a):
@@ -388,10 +389,11 @@ def main():
     pairs_per_s = prof["prop_pairs"] / (prof["prop_ms"] * 1e-3) if prof["prop_ms"] > 0 else 0.0
 
     traffic = None
-    if n == 32 and L == 5 and args.group in (0, 2):
-        # beams of a medium go in groups of two, the odd one alone
-        pairs = (args.beams // 2) * 2
-        per_medium = pairs * TRAFFIC_PER_SOLVE_TERM[2] + (args.beams - pairs) * TRAFFIC_PER_SOLVE_TERM[1]
+    gsz = args.group or DEFAULT_GROUP
+    full, rem = divmod(args.beams, gsz)
+    if n == 32 and L == 5 and (full == 0 or gsz in TRAFFIC_PER_SOLVE_TERM) and (rem == 0 or rem in TRAFFIC_PER_SOLVE_TERM):
+        # the beams of a medium go in groups of gsz, the remainder in one group
+        per_medium = full * gsz * TRAFFIC_PER_SOLVE_TERM.get(gsz, 0.0) + rem * TRAFFIC_PER_SOLVE_TERM.get(rem, 0.0)
         traffic = per_medium * args.media * terms_mean / launches * args.steps
     out = {}
     # error against an all-FP64 solve of medium 0's beams (same term cap)

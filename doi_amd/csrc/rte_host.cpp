@@ -318,7 +318,7 @@ struct rte_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool prof = false;
-    int max_rhs = 2;  // beams per propagation group (1..kMaxNR); 2 measured fastest at 32^3 L=5
+    int max_rhs = 3;  // beams per propagation group (1..kMaxNR); 3 measured fastest at 32^3 L=5
     rte_prof pc{};
     std::vector<hipEvent_t> ev_pool;
     std::vector<PendingTiming> pending;
