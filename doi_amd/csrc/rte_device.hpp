@@ -169,6 +169,92 @@ __device__ __forceinline__ void pair_apply(T (&acc)[(L + 1) * (L + 1)], const T 
     pair_axpy<T, L>(acc, kf * pair_dot<T, L>(w, G), G);
 }
 
+// ---------------------------------------------------------------------------
+// The same pair in expanded form for the propagation kernel: the direction's
+// scaled real harmonics Rt_a(u) are formed once per pair,
+//   Rt_l0 = Pt_l^0,  Rt_l,m = C_m Pt_l^m,  Rt_l,-m = S_m Pt_l^m   (Rt_00 = 1),
+// the products by the Legendre recurrence in l itself (it is linear, so it
+// holds for C_m Pt_l^m with the same gamma):
+//   X_m,m = C_m,  X_m+1,m = z C_m,  X_l,m = z X_l-1,m - gamma_lm X_l-2,m,
+// and every right-hand side then costs one 36-term dot and one axpy of FMAs,
+// with no per-side products C_m t, S_m t or partial-sum set-up.
+// ---------------------------------------------------------------------------
+template <typename T, int L>
+struct PairRt {
+    T R[(L + 1) * (L + 1)];  // R[0] = 1 is not used
+};
+
+template <typename T, int L>
+__device__ __forceinline__ void pair_rt(PairRt<T, L> &G, T x, T y, T z, const ShConst<T> &sh)
+{
+    if constexpr (L >= 1) {
+        G.R[2] = z;
+        T p2 = T(1), p1 = z;
+#pragma unroll
+        for (int l = 2; l <= L; ++l) {
+            const T p = z * p1 - sh.gamma[l][0] * p2;
+            G.R[l * l + l] = p;
+            p2 = p1;
+            p1 = p;
+        }
+        T c = x, s = y;  // C_1 + i S_1 = x + i y
+#pragma unroll
+        for (int m = 1; m <= L; ++m) {
+            if (m > 1) {
+                const T cn = c * x - s * y, sn = s * x + c * y;
+                c = cn;
+                s = sn;
+            }
+            G.R[m * m + m + m] = c;
+            G.R[m * m + m - m] = s;
+            if (m + 1 <= L) {
+                T c2 = c, s2 = s, c1 = z * c, s1 = z * s;
+                G.R[(m + 1) * (m + 1) + (m + 1) + m] = c1;
+                G.R[(m + 1) * (m + 1) + (m + 1) - m] = s1;
+#pragma unroll
+                for (int l = m + 2; l <= L; ++l) {
+                    const T cl = z * c1 - sh.gamma[l][m] * c2, sl = z * s1 - sh.gamma[l][m] * s2;
+                    G.R[l * l + l + m] = cl;
+                    G.R[l * l + l - m] = sl;
+                    c2 = c1;
+                    c1 = cl;
+                    s2 = s1;
+                    s1 = sl;
+                }
+            }
+        }
+    }
+}
+
+// p = sum_a Rt_a w'_a, in two interleaved chains
+template <typename T, int L>
+__device__ __forceinline__ T pair_dot_rt(const T (&w)[(L + 1) * (L + 1)], const PairRt<T, L> &G)
+{
+    constexpr int K = (L + 1) * (L + 1);
+    if constexpr (K == 1) {
+        return w[0];
+    } else if constexpr (K < 3) {
+        return w[0] + G.R[1] * w[1];
+    } else {
+        T p = G.R[1] * w[1], q = G.R[2] * w[2];
+#pragma unroll
+        for (int a = 3; a < K; a += 2) {
+            p += G.R[a] * w[a];
+            if (a + 1 < K) q += G.R[a + 1] * w[a + 1];
+        }
+        return (p + w[0]) + q;
+    }
+}
+
+// acc' += t Rt
+template <typename T, int L>
+__device__ __forceinline__ void pair_axpy_rt(T (&acc)[(L + 1) * (L + 1)], T t, const PairRt<T, L> &G)
+{
+    acc[0] += t;
+#pragma unroll
+    for (int a = 1; a < (L + 1) * (L + 1); ++a) acc[a] += t * G.R[a];
+}
+
 // Walks the same Siddon crossings as march_tau and calls f(v, length) for
 // every voxel v crossed (length = r * dalpha, the destination's half step
 // included): the transpose of the path integral, used to back-project.
