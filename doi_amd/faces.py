@@ -107,6 +107,31 @@ def face_fluxes(field, shape, bounds, L, c=2.9979e10, n=1.0, num=NUM):
     return outward, inward
 
 
+# ComputeSideFaceImages' files (CPU package/src/Dist.c:409, 446, 485, 522)
+SIDE_FACE_FILES = (("-x", "SideFace1(-X).out"), ("+x", "SideFace2(+X).out"), ("-y", "SideFace3(-Y).out"),
+                   ("+y", "SideFace4(+Y).out"))
+
+
+def side_face_images(field, shape, bounds, L, c=2.9979e10, n=1.0, num=NUM):
+    """The four side-face images as ComputeSideFaceImages lays them out: the
+    outward flux of each boundary voxel, -x/+x as [j][k] and -y/+y as [i][k]
+    (k, the z index, fastest; CPU package/src/Dist.c:391-406, 428-443, 467-482,
+    504-519)."""
+    out, _ = face_fluxes(field, shape, bounds, L, c, n, num)
+    return {f: np.ascontiguousarray(out[f].T) for f, _ in SIDE_FACE_FILES}
+
+
+def write_side_faces(directory, images):
+    """The reference's four side-face files in `directory`, in the OutputImage
+    format with each face's own dimensions as the header (the reference writes
+    nX, nY there, the same numbers on a cube)."""
+    import os
+
+    from . import refio
+    for face, name in SIDE_FACE_FILES:
+        refio.write_image(os.path.join(directory, name), images[face])
+
+
 def photon_number(field, shape, bounds, L, ballistic_total, c=2.9979e10, n=1.0, num=NUM):
     """The reference's photon total: outward plus inward flux through the six
     faces plus the ballistic transmission (CPU package/src/Dist.c:538-988,

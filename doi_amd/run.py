@@ -11,14 +11,17 @@ and the top-face reflectance are written to [Runtime] TransFile and ReflectFile
 in the OutputImage format, the phase image to PhaseFile (degrees,
 ComputePhaseImage, CPU package/src/Dist.c:71-127: the argument of the exitance
 with the scattered part's quadrature divided by 99 instead of 100); with --out
-only the transmitted image is written, there.  The photon total (outward and
-inward flux through the six faces plus the ballistic transmission,
-CPU package/src/Dist.c:538-988) is printed as the reference prints it.
+only the transmitted image is written, there.  Like the reference, the command
+line also writes the four side-face images SideFace1(-X).out .. SideFace4(+Y).out
+to the working directory (ComputeSideFaceImages, CPU package/src/Dist.c:353-530;
+--no-side-faces skips them).  The photon total (outward and inward flux through
+the six faces plus the ballistic transmission, CPU package/src/Dist.c:538-988)
+is printed as the reference prints it.
 
 A [Source] modulation (angular frequency, rad/s) adds the frequency-domain
 solve (rte_solve_fd) of CPU package/src/RTE_uniform.c:120-177: PhaseFile is the
-phase of its complex exitance, and TransFile, ReflectFile and the photon total
-are those of CombineOuts' field (Dist.c:1294-1303), half the continuous-wave
+phase of its complex exitance, and TransFile, ReflectFile, the side faces and the
+photon total are those of CombineOuts' field (Dist.c:1294-1303), half the continuous-wave
 field plus half the real part of the modulated one; imaginary planes are 0.
 """
 import argparse
@@ -53,7 +56,8 @@ def phase_image(exitance, ballistic):
     return np.angle(ballistic + (exitance - ballistic) * (100.0 / 99.0)) * 180.0 / np.pi
 
 
-def solve_file(path, precision="mixed", fixed_terms=False, tol=None, out=None, ctx=None, quiet=False):
+def solve_file(path, precision="mixed", fixed_terms=False, tol=None, out=None, ctx=None, quiet=False,
+               side_faces=False):
     prob = refio.load(path)
     if fixed_terms:
         params = rte.make_params(prob.L, precision=PRECISIONS[precision], stop_rule=rte.STOP_DELTA, tol=0.0,
@@ -92,6 +96,7 @@ def solve_file(path, precision="mixed", fixed_terms=False, tol=None, out=None, c
             np.ascontiguousarray(field_c.real), prob.shape, prob.bounds, prob.L, float(np.sum(ball_c.real)),
             c=prob.c, n=prob.n)
         st["terms_fd"] = st_c["terms"]
+        field = 0.5 * field + 0.5 * field_c.real
     st["phase"] = phase
     dest = out or prob.trans_file
     if dest:
@@ -101,6 +106,9 @@ def solve_file(path, precision="mixed", fixed_terms=False, tol=None, out=None, c
             refio.write_image(prob.reflect_file, refl)
         if prob.phase_file:
             refio.write_phase_image(prob.phase_file, phase)
+        if side_faces:
+            faces.write_side_faces(".", faces.side_face_images(field, prob.shape, prob.bounds, prob.L, c=prob.c,
+                                                               n=prob.n))
     if not quiet:
         print("Number of photons total: %f" % st["photons"])
         print("Neumann series: %d" % st.get("terms_fd", st["terms"]))
@@ -114,8 +122,10 @@ def main(argv=None):
     ap.add_argument("--fixed-terms", action="store_true")
     ap.add_argument("--tol", type=float, default=None)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--no-side-faces", action="store_true",
+                    help="do not write SideFace1(-X).out .. SideFace4(+Y).out to the working directory")
     a = ap.parse_args(argv)
-    solve_file(a.ini, a.precision, a.fixed_terms, a.tol, a.out)
+    solve_file(a.ini, a.precision, a.fixed_terms, a.tol, a.out, side_faces=not a.no_side_faces)
     return 0
 
 

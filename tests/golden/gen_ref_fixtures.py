@@ -7,7 +7,8 @@ from its own sources by oracle/Makefile.ref into oracle/_ref/rte_cpu.
 Run in the build container only (it needs /root/reference to build the
 binary):  make -f oracle/Makefile.ref && python tests/golden/gen_ref_fixtures.py
 The fixture holds data only: the .par inputs and the binary's outputs
-(transmitted and reflected images, term count, printed photon total).
+(transmitted and reflected images, term count, printed photon total, and for
+cubic grids the four side-face images).
 """
 import json
 import os
@@ -101,6 +102,18 @@ def run_case(case, mod=0.0):
             images[key] = vals[: rx * ry].reshape(ry, rx).tolist()
             if mod:
                 images[key + "_imag"] = vals[rx * ry: 2 * rx * ry].reshape(ry, rx).tolist()
+        if not mod and nx == ny == nz:
+            # ComputeSideFaceImages writes these under the OutputImage header (nX, nY):
+            # only a cube gives the face's own shape (CPU package/src/Dist.c:409-522)
+            side = {}
+            for face, fname in (("-x", "SideFace1(-X).out"), ("+x", "SideFace2(+X).out"),
+                                ("-y", "SideFace3(-Y).out"), ("+y", "SideFace4(+Y).out")):
+                raw = open(os.path.join(d, fname), "rb").read()
+                rx, ry = struct.unpack("ii", raw[:8])
+                vals = np.frombuffer(raw[8:], dtype=np.float64)
+                side[face] = vals[: rx * ry].reshape(ry, rx).tolist()
+                side[face + "_imag_max"] = float(np.max(np.abs(vals[rx * ry: 2 * rx * ry])))
+            images["side_faces"] = side
         if mod:
             raw = open(os.path.join(d, "phase.out"), "rb").read()
             rx, ry = struct.unpack("ii", raw[:8])
@@ -131,7 +144,9 @@ def main():
     with open(os.path.join(HERE, "ref_cpu_uniform.json"), "w") as f:
         json.dump(dict(source="reference CPU package (CPU package/src), built by oracle/Makefile.ref",
                        image_layout="trans_image / refl_image [j][i], j = y index, i = x index (OutputImage "
-                                    "order); photons_printed: the binary's 'Number of photons total' (%f)",
+                                    "order); photons_printed: the binary's 'Number of photons total' (%f); "
+                                    "side_faces (cubic grids): SideFace1..4 real planes, -x/+x [j][k], "
+                                    "-y/+y [i][k] (k = z index fastest), with the largest imaginary value",
                        cases=res), f, indent=1)
     for r in res:
         print(r["name"], r["terms"], np.sum(r["trans_image"]))

@@ -2,8 +2,8 @@
 the transmitted image written in the reference's OutputImage format, and the
 reflectance image, equal the reference CPU package's for the same file (FP64
 within 1e-9, same lambda-rule term count); the command-line entry with a fixed
-term count writes both files; a modulated source adds the frequency-domain
-solve to the files."""
+term count writes both files and the four side-face files; a modulated source
+adds the frequency-domain solve to the files."""
 import numpy as np
 import pytest
 
@@ -43,10 +43,13 @@ def test_parameter_file_end_to_end(ctx, golden_ref, tmp_path):
         assert run.main([str(p), "--fixed-terms"]) == 0
         t, _ = refio.read_image("trans.out")
         r, _ = refio.read_image("refl.out")
+        side = {f: refio.read_image(name)[0] for f, name in faces.SIDE_FACE_FILES}
     finally:
         os.chdir(cwd)
     assert rel_max(t, np.array(case["trans_image"])) < 1e-4
     assert rel_max(r, np.array(case["refl_image"])) < 1e-4
+    for f, _ in faces.SIDE_FACE_FILES:   # the reference binary's SideFace1..4 files
+        assert rel_max(side[f], np.array(case["side_faces"][f])) < 1e-4, f
 
 
 def test_modulated_parameter_file(ctx, golden_ref, golden_mod, tmp_path):

@@ -171,3 +171,31 @@ def test_face_quadratures_reproduce_reference_photon_totals(golden_ref):
         tot = faces.photon_number(r["w"], tuple(c["shape"]), tuple(c["bounds"]), c["L"], r["ballistic_total"],
                                   n=c["n"])
         assert abs(tot - c["photons_printed"]) <= 5.0e-7 + 1e-12, c["name"]
+
+
+def test_side_face_images_reproduce_reference_files(golden_ref, tmp_path):
+    """doi_amd.faces.side_face_images on the oracle's final field against the
+    SideFace1..4 files the reference binary wrote (cubic grids, where its header
+    is the face's shape), and the files written back in that layout."""
+    from doi_amd import faces
+    n_checked = 0
+    for c in golden_ref:
+        if "side_faces" not in c:
+            continue
+        b = c["beam"]
+        P = O.Problem(tuple(c["shape"]), tuple(c["bounds"]), c["mua"], c["mus"], c["g"], n=c["n"])
+        r = P.solve((b["pos_x"], b["pos_y"], b["diam"], b["mag"]), c["L"], prop_thresh=c["prop_thresh"],
+                    field=True)
+        imgs = faces.side_face_images(r["w"], tuple(c["shape"]), tuple(c["bounds"]), c["L"], n=c["n"])
+        for face, _ in faces.SIDE_FACE_FILES:
+            ref = np.array(c["side_faces"][face])
+            assert c["side_faces"][face + "_imag_max"] < 1e-15
+            assert imgs[face].shape == ref.shape
+            assert rel_max(imgs[face], ref) < 1e-10, (c["name"], face)
+        n_checked += 1
+        if n_checked == 1:
+            faces.write_side_faces(str(tmp_path), imgs)
+            for face, name in faces.SIDE_FACE_FILES:
+                re_, im = refio.read_image(str(tmp_path / name))
+                assert np.array_equal(re_, imgs[face]) and not np.any(im)
+    assert n_checked == 6
