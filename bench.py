@@ -38,6 +38,90 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+REF_INI = """[Runtime]
+TransFile = trans.out
+ReflectFile = refl.out
+PhaseFile = phase.out
+[Algorithm]
+nL = {L}
+nTerms = 30
+[Geometry]
+nX = {n}
+nY = {n}
+nZ = {n}
+xMin = {a!r}
+xMax = {b!r}
+yMin = {a!r}
+yMax = {b!r}
+zMin = 0.0
+zMax = {z!r}
+subVox = 0
+subThresh = 0.0
+propThresh = 10.0
+selfSubVox = 1
+[Phantom]
+mua = 0.01
+mus = 1.0
+g = 0.9
+n = 1.0
+[Source]
+posX = 0.0
+posY = 0.0
+diam = 0.2
+magnitude = 1.0
+modulation = 0.0
+"""
+
+
+def reference_cpu_propagation(n=10, L=5):
+    """The reference CPU package itself (built from its sources into
+    oracle/_ref/rte_cpu; absent -> None) on a uniform n^3 medium of the
+    workload's optics: seconds per propagation A, from the times its progress
+    lines appear (read through a pty, so they are line-buffered): the loop from
+    "Neuman series iteration" to "Neumann series: n" holds n - 1 propagations
+    and scatters plus the final propagation.  Run before this process touches
+    the GPU."""
+    import pty
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "oracle", "_ref", "rte_cpu")
+    if not os.access(exe, os.X_OK):
+        return None
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "in.ini"), "w") as f:
+            f.write(REF_INI.format(L=L, n=n, a=-0.05 * n, b=0.05 * n, z=0.1 * n))
+        master, slave = pty.openpty()
+        t0 = time.perf_counter()
+        proc = subprocess.Popen([exe, "in.ini"], cwd=d, stdout=slave, stderr=subprocess.DEVNULL,
+                                env=dict(os.environ, OMP_NUM_THREADS=str(cores)))
+        os.close(slave)
+        buf, stamps = b"", []
+        while True:
+            try:
+                chunk = os.read(master, 4096)
+            except OSError:
+                break
+            if not chunk:
+                break
+            now = time.perf_counter() - t0
+            buf += chunk
+            while b"\n" in buf:
+                line, buf = buf.split(b"\n", 1)
+                stamps.append((now, line.decode(errors="replace").strip()))
+        rc = proc.wait()
+        os.close(master)
+    try:
+        t_loop = [t for t, l in stamps if "Neuman series iteration" in l][0]
+        t_done, done = [(t, l) for t, l in stamps if l.startswith("Neumann series:")][0]
+        nprop = int(done.split(":")[1])
+    except (IndexError, ValueError):
+        log("reference CPU run gave no timing (rc %d)" % rc)
+        return None
+    return dict(per_prop_s=(t_done - t_loop) / nprop, n=n, L=L, propagations=nprop, cores=cores,
+                total_s=stamps[-1][0] if stamps else 0.0)
+
+
 def cpu_baseline(shape, bounds, mu_a, mu_s, g, L, terms, budget_s=12.0):
     """FP64 oracle (C, OpenMP) timed on this host on a sample of destination
     voxels of one propagation; scaled to solves/s with the GPU's term count."""
@@ -338,6 +422,10 @@ def main():
         return config3(args)
     if args.fd:
         return frequency_domain(args)
+    ref_cpu = None
+    if not args.no_cpu:
+        log("reference CPU package (uniform 10^3 L=5 sample)")
+        ref_cpu = reference_cpu_propagation(10, args.L)
     from doi_amd import media, rte
 
     n, L = args.n, args.L
@@ -444,7 +532,23 @@ def main():
     res["ktab_ms_total"] = prof["ktab_ms"]
     if not args.no_cpu:
         log("cpu baseline")
-        res["cpu_baseline"] = cpu_baseline(shape, bounds, maps[0][0], maps[0][1], g, L, terms_mean)
+        port = cpu_baseline(shape, bounds, maps[0][0], maps[0][1], g, L, terms_mean)
+        if ref_cpu is not None:
+            # the reference's propagation is O(pairs): scale the sample by the pair count
+            scale = (float(n) ** 3 / float(ref_cpu["n"]) ** 3) ** 2
+            per_prop = ref_cpu["per_prop_s"] * scale
+            res["cpu_baseline"] = dict(
+                value=1.0 / (per_prop * terms_mean), unit="solves/s", cores=ref_cpu["cores"], kind="reference",
+                sample="reference CPU package (CPU package/src, built from its sources) on a uniform %d^3 L=%d "
+                       "medium of the workload's optics: %.3f s per propagation (%d propagations, %.1f s run), "
+                       "scaled by the voxel-pair count (x%.0f) to %d^3 = %.0f s per propagation, x %.1f Neumann "
+                       "terms per solve; its per-pair cost grows with the grid (the displacement table leaves the "
+                       "caches), so the scaling favours the CPU"
+                       % (ref_cpu["n"], ref_cpu["L"], ref_cpu["per_prop_s"], ref_cpu["propagations"],
+                          ref_cpu["total_s"], scale, n, per_prop, terms_mean))
+            res["cpu_baseline_port"] = port
+        else:
+            res["cpu_baseline"] = port
     print(json.dumps(res), flush=True)
     for m in mm:
         m.close()
