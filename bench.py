@@ -8,9 +8,9 @@ mu_a inclusions, 5 pencil beams per medium -> 160 solves, Neumann series to a
 in HBM before the timed region; every step rebuilds their path tables.
 
     python bench.py [--gpus 1] [--steps 2] [--warmup 1]
-    python bench.py --serve [--requests 2000] [--rate 200]   # serving stream (config 4)
-    python bench.py --config3                                # gradient + Fisher at 24^3
-    python bench.py --fd [--omega 6.28e8]                    # config 5 with modulated beams
+    python bench.py --serve [--requests 2000] [--rate 200]   # serving stream (2000 requests, 200/s)
+    python bench.py --gradient                                # gradient + Fisher at 24^3
+    python bench.py --fd [--omega 6.28e8]                    # the batch with modulated beams
 
 Prints one JSON line.  One GPU only: --gpus N > 1 exits with a message.
 """
@@ -227,8 +227,8 @@ def serve(args):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32 (fp64 running sums)",
-        "data": "synthetic (seeded serving stream, config 4)",
-        "config": {"workload": "config4: %d requests, Poisson %.0f/s, edges {8,12,16,24,32}, L {3,5}, "
+        "data": "synthetic (seeded serving stream)",
+        "config": {"workload": "serving stream: %d requests, Poisson %.0f/s, edges {8,12,16,24,32}, L {3,5}, "
                                "g U[0.7,0.9], mu_s U[0.5,1.5], mu_a U[0.005,0.05], sigma 0.2, tol 1e-6 cap 30"
                                % (len(reqs), args.rate)},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
@@ -271,13 +271,14 @@ def cpu_baseline_fd(shape, bounds, mu_a, mu_s, g, L, omega, terms, budget_s=12.0
 
 
 def frequency_domain(args):
-    """The config-5 batch with every beam intensity-modulated (omega rad/s):
+    """The batch of 32 lognormal media x 5 beams at 32^3, L=5, with every beam
+    intensity-modulated (omega rad/s):
     complex Neumann series through the complex propagation group."""
     from doi_amd import media, rte
     n, L, g = args.n, args.L, 0.9
     shape, bounds = (n, n, n), media.cube_bounds(n)
-    maps = media.config5_media(args.media, seed=5, n=n)
-    beams5 = media.config5_beams(seed=5, n=n, per_medium=args.beams)
+    maps = media.lognormal_media_with_inclusions(args.media, seed=5, n=n)
+    beams5 = media.top_face_beams(seed=5, n=n, per_medium=args.beams)
     ctx = rte.Context(0)
     mm = [ctx.medium(shape, bounds, ma, ms, g) for ma, ms in maps]
     med_list = [m for m in mm for _ in range(args.beams)]
@@ -315,8 +316,8 @@ def frequency_domain(args):
            "value": nsolve / (ms_per_step / 1000.0), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "fp32 complex (fp64 running sums)",
-           "data": "synthetic (seeded lognormal media, config 5)",
-           "config": {"workload": "config5 media x beams, %d^3, L=%d, every beam modulated at omega = %.4g rad/s"
+           "data": "synthetic (seeded lognormal media with inclusions)",
+           "config": {"workload": "32 lognormal media with inclusions x 5 beams, %d^3, L=%d, every beam modulated at omega = %.4g rad/s"
                                   % (n, L, args.omega), "solves_per_step": nsolve},
            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
            "terms_mean": terms_mean, "rel_err_vs_fp64": err,
@@ -333,7 +334,7 @@ def frequency_domain(args):
     ctx.close()
 
 
-def config3(args):
+def gradient_and_fisher(args):
     """Differentiation workload: 24^3 heterogeneous medium, L=5, mixed
     precision; the loss gradient with respect to every voxel's mu_a and mu_s
     (one forward + one adjoint series) and the Fisher matrix of 64 block
@@ -365,11 +366,11 @@ def config3(args):
         tf.append(time.perf_counter() - t0)
         log("gradient %.3f s, Fisher %.3f s" % (tg[-1], tf[-1]))
     ev = np.linalg.eigvalsh(F)
-    res = {"metric": "config3: gradient and Fisher time at 24^3 L=5", "value": 1.0 / float(np.mean(tg)),
+    res = {"metric": "gradient and Fisher time at 24^3 L=5", "value": 1.0 / float(np.mean(tg)),
            "unit": "gradients/s", "n_gpus": 1, "steps": len(tg), "warmup": 1,
            "ms_per_step": 1000.0 * float(np.mean(tg)), "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "fp32 (fp64 running sums)", "data": "synthetic (seeded, config 3)",
-           "config": {"workload": "config3: 24^3, L=5, g=0.8, loss vs second seeded map; Fisher 64 mu_a blocks "
+           "vs_baseline": None, "dtype": "fp32 (fp64 running sums)", "data": "synthetic (seeded lognormal map with an inclusion)",
+           "config": {"workload": "24^3, L=5, g=0.8, loss vs second seeded map; Fisher 64 mu_a blocks "
                                   "(6^3 voxels), 576 detectors"},
            "terms": st["terms"], "gradient_s": float(np.mean(tg)), "fisher_s": float(np.mean(tf)),
            "fisher_eig_min_max": [float(ev[0]), float(ev[-1])]}
@@ -394,7 +395,7 @@ def main():
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--plain-sample", type=int, default=2)
-    ap.add_argument("--config3", action="store_true", help="time the gradient and Fisher workload instead")
+    ap.add_argument("--gradient", action="store_true", help="time the gradient and Fisher workload instead")
     ap.add_argument("--fd", action="store_true", help="time the frequency-domain (modulated beam) workload")
     ap.add_argument("--omega", type=float, default=2.0 * np.pi * 1.0e8, help="modulation, rad/s (--fd)")
     ap.add_argument("--calib-copy", type=int, default=0,
@@ -418,8 +419,8 @@ def main():
 
     if args.serve:
         return serve(args)
-    if args.config3:
-        return config3(args)
+    if args.gradient:
+        return gradient_and_fisher(args)
     if args.fd:
         return frequency_domain(args)
     ref_cpu = None
@@ -432,8 +433,8 @@ def main():
     shape, bounds = (n, n, n), media.cube_bounds(n)
     g = 0.9
     log("generating %d media of %d^3" % (args.media, n))
-    maps = media.config5_media(args.media, seed=5, n=n)
-    beams5 = media.config5_beams(seed=5, n=n, per_medium=args.beams)
+    maps = media.lognormal_media_with_inclusions(args.media, seed=5, n=n)
+    beams5 = media.top_face_beams(seed=5, n=n, per_medium=args.beams)
     ctx = rte.Context(0)
     if args.group:
         ctx.set_group_size(args.group)
@@ -515,8 +516,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32 (fp64 running sums)",
-        "data": "synthetic (seeded lognormal media, config 5)",
-        "config": {"workload": "config5: %d media x %d beams, %d^3 voxels 0.1 cm, L=%d, g=0.9, tol 1e-6 cap 30"
+        "data": "synthetic (seeded lognormal media with inclusions)",
+        "config": {"workload": "%d lognormal media with inclusions x %d beams, %d^3 voxels 0.1 cm, L=%d, g=0.9, tol 1e-6 cap 30"
                                % (args.media, args.beams, n, L),
                    "solves_per_step": nsolve, "grid": n, "L": L, "precision": "mixed",
                    "path": "cached path integrals"},

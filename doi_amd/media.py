@@ -72,20 +72,20 @@ def heterogeneous(n, seed, mua=0.01, mus=1.0, sigma=0.2, n_incl=1, radius=(3, 3)
     return ma, ms
 
 
-def config1():
+def uniform_8cube():
     """8^3 uniform cube, 0.1 cm voxels, mu_a 0.01, mu_s 1, g 0.8, L 3, centred beam diam 0.2."""
     return dict(shape=(8, 8, 8), bounds=cube_bounds(8), mu_a=0.01, mu_s=1.0, g=0.8, L=3,
                 beam=(0.0, 0.0, 0.2, 1.0))
 
 
-def config2(seed=2):
+def lognormal_16cube(seed=2):
     """16^3 heterogeneous (lognormal sigma 0.2, one radius-3 sphere with mu_a x2), L 5, g 0.8."""
     ma, ms = heterogeneous(16, seed, sigma=0.2, n_incl=1, radius=(3, 3), factor=(2.0, 2.0))
     return dict(shape=(16, 16, 16), bounds=cube_bounds(16), mu_a=ma, mu_s=ms, g=0.8, L=5,
                 beam=(0.0, 0.0, 0.2, 1.0))
 
 
-def config5_media(n_media=32, seed=5, n=32):
+def lognormal_media_with_inclusions(n_media=32, seed=5, n=32):
     """32 media of 32^3 (lognormal sigma 0.3, 1-3 spheres radius 2-5, mu_a x1.5-3), g 0.9."""
     out = []
     for m in range(n_media):
@@ -95,7 +95,7 @@ def config5_media(n_media=32, seed=5, n=32):
     return out
 
 
-def config5_beams(seed=5, n=32, per_medium=5, dx=0.1):
+def top_face_beams(seed=5, n=32, per_medium=5, dx=0.1):
     """Pencil beams (diam = one voxel) at seeded top-face voxel centres."""
     st = Stream(seed * 7919 + 1)
     xs = voxel_centres(n, dx)

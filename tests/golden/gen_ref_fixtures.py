@@ -25,12 +25,12 @@ BIN = os.path.join(REPO, "oracle", "_ref", "rte_cpu")
 
 # name, grid (nx,ny,nz), bounds, mua, mus, g, n, L, beam (posX,posY,diam), propThresh
 CASES = [
-    ("config1_L3_g0.8", (8, 8, 8), (-0.4, 0.4, -0.4, 0.4, 0.0, 0.8), 0.01, 1.0, 0.8, 1.0, 3, (0.0, 0.0, 0.2), 10.0),
-    ("config1_L0_g0.8", (8, 8, 8), (-0.4, 0.4, -0.4, 0.4, 0.0, 0.8), 0.01, 1.0, 0.8, 1.0, 0, (0.0, 0.0, 0.2), 10.0),
-    ("config1_L1_g0.8", (8, 8, 8), (-0.4, 0.4, -0.4, 0.4, 0.0, 0.8), 0.01, 1.0, 0.8, 1.0, 1, (0.0, 0.0, 0.2), 10.0),
-    ("config1_L5_g0.8", (8, 8, 8), (-0.4, 0.4, -0.4, 0.4, 0.0, 0.8), 0.01, 1.0, 0.8, 1.0, 5, (0.0, 0.0, 0.2), 10.0),
-    ("config1_L3_g0", (8, 8, 8), (-0.4, 0.4, -0.4, 0.4, 0.0, 0.8), 0.01, 1.0, 0.0, 1.0, 3, (0.0, 0.0, 0.2), 10.0),
-    ("config1_L1_g0", (8, 8, 8), (-0.4, 0.4, -0.4, 0.4, 0.0, 0.8), 0.01, 1.0, 0.0, 1.0, 1, (0.0, 0.0, 0.2), 10.0),
+    ("cube8_L3_g0.8", (8, 8, 8), (-0.4, 0.4, -0.4, 0.4, 0.0, 0.8), 0.01, 1.0, 0.8, 1.0, 3, (0.0, 0.0, 0.2), 10.0),
+    ("cube8_L0_g0.8", (8, 8, 8), (-0.4, 0.4, -0.4, 0.4, 0.0, 0.8), 0.01, 1.0, 0.8, 1.0, 0, (0.0, 0.0, 0.2), 10.0),
+    ("cube8_L1_g0.8", (8, 8, 8), (-0.4, 0.4, -0.4, 0.4, 0.0, 0.8), 0.01, 1.0, 0.8, 1.0, 1, (0.0, 0.0, 0.2), 10.0),
+    ("cube8_L5_g0.8", (8, 8, 8), (-0.4, 0.4, -0.4, 0.4, 0.0, 0.8), 0.01, 1.0, 0.8, 1.0, 5, (0.0, 0.0, 0.2), 10.0),
+    ("cube8_L3_g0", (8, 8, 8), (-0.4, 0.4, -0.4, 0.4, 0.0, 0.8), 0.01, 1.0, 0.0, 1.0, 3, (0.0, 0.0, 0.2), 10.0),
+    ("cube8_L1_g0", (8, 8, 8), (-0.4, 0.4, -0.4, 0.4, 0.0, 0.8), 0.01, 1.0, 0.0, 1.0, 1, (0.0, 0.0, 0.2), 10.0),
     ("box_6x5x4_L2_offcentre_n1.37", (6, 5, 4), (-0.3, 0.3, -0.2, 0.3, 0.0, 0.48), 0.05, 2.0, 0.5, 1.37, 2,
      (0.1, 0.05, 0.15), 10.0),
     ("slab_7x7x3_L4_g0.9_wide_beam", (7, 7, 3), (-0.35, 0.35, -0.35, 0.35, 0.0, 0.3), 0.1, 3.0, 0.9, 1.0, 4,
@@ -40,8 +40,8 @@ CASES = [
 # modulated sources (CPU package/src/RTE_uniform.c:120-177): name, case of CASES it
 # modifies, modulation (angular frequency, rad/s)
 MOD_CASES = [
-    ("config1_L3_g0.8_mod6.3e9", "config1_L3_g0.8", 6.283185307179586e9),
-    ("config1_L1_g0_mod3e10", "config1_L1_g0", 3.0e10),
+    ("cube8_L3_g0.8_mod6.3e9", "cube8_L3_g0.8", 6.283185307179586e9),
+    ("cube8_L1_g0_mod3e10", "cube8_L1_g0", 3.0e10),
     ("box_6x5x4_L2_offcentre_n1.37_mod2e10", "box_6x5x4_L2_offcentre_n1.37", 2.0e10),
 ]
 

@@ -74,9 +74,9 @@ def test_media_are_seeded_and_shaped():
     # log-normal factors with sigma 0.2 around the background
     z = np.log(s1 / 1.0)
     assert abs(z.mean()) < 0.02 and abs(z.std() - 0.2) < 0.02
-    c5 = media.config5_media(2, seed=5, n=12)
+    c5 = media.lognormal_media_with_inclusions(2, seed=5, n=12)
     assert len(c5) == 2 and c5[0][0].shape == (1728,)
-    beams = media.config5_beams(seed=5, n=32)
+    beams = media.top_face_beams(seed=5, n=32)
     assert len(beams) == 5 and all(abs(b[2] - 0.1) < 1e-15 for b in beams)
 
 

@@ -108,8 +108,8 @@ def test_fd_zero_frequency_conjugate_symmetry_and_batch(ctx):
         m.close()
 
 
-def test_fd_config2_size_against_oracle(ctx):
-    """16^3, L = 5 (the lognormal config-2 grid) at 1 GHz: FP64 and mixed paths
+def test_fd_16cube_L5_against_oracle(ctx):
+    """16^3, L = 5 (a lognormal grid) at 1 GHz: FP64 and mixed paths
     against the oracle's complex series (about 15 s of oracle time)."""
     shape, L, om = (16, 16, 16), 5, 2.0 * np.pi * 1.0e9
     bounds = bounds_for(shape)

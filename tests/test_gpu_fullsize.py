@@ -1,4 +1,4 @@
-"""GPU parity at the workloads' full sizes: config 2 (16^3, L=5) against the
+"""GPU parity at the workloads' full sizes: the 16^3 L=5 lognormal medium against the
 committed FP64 oracle fixture, and size-independent properties at 32^3
 (mixed vs FP64 error bar, mirror symmetry, linearity in the beam power,
 batch independence)."""
@@ -16,17 +16,17 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 @pytest.fixture(scope="module")
-def config2_fixture():
-    with open(os.path.join(HERE, "golden", "oracle_config2.json")) as f:
+def lognormal_16cube_fixture():
+    with open(os.path.join(HERE, "golden", "oracle_lognormal_16cube_L5.json")) as f:
         return json.load(f)
 
 
 @pytest.mark.parametrize("prec,tol", [(rte.PREC_FP64, 1e-9), (rte.PREC_MIXED, 1e-5), (rte.PREC_FP32, 1e-4)])
-def test_config2_against_oracle_fixture(ctx, config2_fixture, prec, tol):
-    fx = config2_fixture
-    c = media.config2(seed=fx["seed"])
+def test_lognormal_16cube_against_oracle_fixture(ctx, lognormal_16cube_fixture, prec, tol):
+    fx = lognormal_16cube_fixture
+    c = media.lognormal_16cube(seed=fx["seed"])
     for case in fx["cases"]:
-        print("config2", case["name"], prec, flush=True)
+        print("16^3 L=5", case["name"], prec, flush=True)
         p = rte.make_params(fx["L"], precision=prec, stop_rule=case["stop_rule"], tol=case["tol"],
                             max_terms=case["max_terms"])
         img, st = ctx.solve(c["shape"], c["bounds"], c["mu_a"], c["mu_s"], c["g"], c["beam"], p)
@@ -36,9 +36,9 @@ def test_config2_against_oracle_fixture(ctx, config2_fixture, prec, tol):
 
 def test_32cube_mixed_within_1e4_of_fp64(ctx):
     n, L = 32, 5
-    ma, ms = media.config5_media(1, seed=5, n=n)[0]
+    ma, ms = media.lognormal_media_with_inclusions(1, seed=5, n=n)[0]
     m = ctx.medium((n, n, n), media.cube_bounds(n), ma, ms, 0.9)
-    beams = media.config5_beams(seed=5, n=n)[:2]
+    beams = media.top_face_beams(seed=5, n=n)[:2]
     out = {}
     for prec in (rte.PREC_FP64, rte.PREC_MIXED, rte.PREC_FP32):
         print("32^3", prec, flush=True)

@@ -30,7 +30,7 @@ def _maps(seed):
     return media.heterogeneous(n, seed, sigma=0.2, n_incl=1, radius=(3, 3), factor=(2.0, 2.0))
 
 
-def test_config3_adjoint_gradient_against_finite_differences(ctx):
+def test_adjoint_gradient_24cube_against_finite_differences(ctx):
     ma, ms = _maps(3)
     ma2, ms2 = _maps(4)
     p64 = rte.make_params(L, precision=rte.PREC_FP64, stop_rule=rte.STOP_DELTA, tol=1e-6, max_terms=30)
@@ -65,14 +65,14 @@ def test_config3_adjoint_gradient_against_finite_differences(ctx):
             out.append((lp - lm) / (2 * h))
     t_fd = time.perf_counter() - t0
     fd_a, fd_s = np.array(fd_a), np.array(fd_s)
-    print("config3: %d terms, adjoint %.2f s for all %d voxels x 2; FD %.2f s for 32 voxels x 2"
+    print("24^3 gradient: %d terms, adjoint %.2f s for all %d voxels x 2; FD %.2f s for 32 voxels x 2"
           % (nt, t_adj, n ** 3, t_fd))
     for adj, fd in ((ga[vox], fd_a), (gs[vox], fd_s)):
         sc = np.max(np.abs(fd))
         assert np.all(np.abs(adj - fd) <= 1e-4 * sc + 1e-3 * np.abs(fd)), np.max(np.abs(adj - fd)) / sc
 
 
-def test_config3_fisher_64_parameters_576_detectors(ctx):
+def test_fisher_24cube_64_parameters_576_detectors(ctx):
     ma, ms = _maps(3)
     med = ctx.medium(SHAPE, BOUNDS, ma, ms, G)
     i, j, k = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
@@ -84,7 +84,7 @@ def test_config3_fisher_64_parameters_576_detectors(ctx):
         t0 = time.perf_counter()
         F, Cf, J, img, st = ctx.fisher(med, BEAM, pm, part, 64, wrt=0, jacobian=jac)
         dt = time.perf_counter() - t0
-        print("config3 Fisher (%s Jacobian): %.2f s, %d terms" % ("reference" if jac else "exact", dt, st["terms"]))
+        print("24^3 Fisher (%s Jacobian): %.2f s, %d terms" % ("reference" if jac else "exact", dt, st["terms"]))
         E = img.ravel()
         assert J.shape == (576, 64) and np.all(E > 0)
         F64 = J.T @ (J / E[:, None])

@@ -131,7 +131,7 @@ def test_admission_budget_cancel_and_back_pressure():
         sctx.close()
 
 
-def test_served_config4_sample_matches_alone(ctx):
+def test_served_stream_sample_matches_alone(ctx):
     """A small seeded sample of the mixed serving stream (grid edges 8-32,
     L 3/5, random g, mu and beam): served results equal the solves run alone."""
     reqs = media.serving_requests(12, seed=4, edges=(8, 12, 16), rate=0.0)

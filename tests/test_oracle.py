@@ -318,13 +318,13 @@ def test_mirror_symmetry_of_centred_beam():
     assert rel_max(e[::-1, :], e) < 1e-12
 
 
-def test_config2_fixture_reproduces():
+def test_lognormal_16cube_fixture_reproduces():
     """The committed 16^3 oracle fixture still matches the oracle (guards the
     fixture the GPU tests use) -- run on a subsampled stop rule to stay fast."""
-    with open(os.path.join(HERE, "golden", "oracle_config2.json")) as f:
+    with open(os.path.join(HERE, "golden", "oracle_lognormal_16cube_L5.json")) as f:
         fx = json.load(f)
     from doi_amd import media
-    c = media.config2(seed=fx["seed"])
+    c = media.lognormal_16cube(seed=fx["seed"])
     assert list(c["shape"]) == fx["shape"]
     lam = [x for x in fx["cases"] if x["name"] == "lambda_ref"][0]
     assert lam["terms"] == 10
