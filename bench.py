@@ -413,7 +413,7 @@ def main():
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (independent solves "
-              "shard across GPUs with no communication; see DESIGN.md). Run with --gpus 1.",
+              "shard across GPUs with no communication). Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
 

@@ -1,8 +1,7 @@
 """Writes tests/golden/oracle_lognormal_16cube_L5.json: the FP64 oracle's exitance for the
 16^3 heterogeneous L=5 workload (seeded maps from doi_amd.media.lognormal_16cube), with
 the reference lambda stop rule and with the 1e-6 total-exitance rule.  The GPU
-tests compare against this file on the GPU box (where the oracle would take
-minutes at this size).
+tests compare against this file (the oracle would take minutes at this size).
 
     python tests/golden/gen_oracle_fixtures.py
 """
