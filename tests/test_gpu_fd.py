@@ -129,3 +129,24 @@ def test_fd_16cube_L5_against_oracle(ctx):
     big = np.abs(ref["exitance"]) > 1e-2 * np.max(np.abs(ref["exitance"]))
     assert np.all(np.angle(img[0][big]) < 0)
     m.close()
+
+
+@pytest.mark.parametrize("L", [0, 4, 6, 7])
+def test_fd_orders_against_oracle(ctx, L):
+    """Every compiled SH order of the complex kernels (the FP64 complex combine
+    spills at L = 7), a small heterogeneous medium, FP64 and mixed paths."""
+    shape, om = (4, 5, 4), 4.0e10
+    bounds = bounds_for(shape)
+    ma, ms = hetero_maps(shape, seed=90 + L, mua=0.03, mus=1.2)
+    beam = (0.0, 0.05, 0.25, 1.0)
+    ref = O.Problem(shape, bounds, ma, ms, 0.85, n=1.3).solve_fd(beam, L, om, stop_rule=O.STOP_DELTA, tol=1e-9,
+                                                                max_terms=25, field=True)
+    m = ctx.medium(shape, bounds, ma, ms, 0.85, n=1.3)
+    for prec in (rte.PREC_FP64, rte.PREC_MIXED):
+        print("fd order", L, prec, flush=True)
+        p = rte.make_params(L, precision=prec, stop_rule=rte.STOP_DELTA, tol=1e-9, max_terms=25)
+        img, refl, st, fld = ctx.solve_fd([m], [beam], [om], p, field=True)
+        assert st[0]["terms"] == ref["terms"]
+        assert crel(img[0], ref["exitance"]) < TOL[prec]
+        assert crel(fld, ref["w"]) < TOL[prec]
+    m.close()
