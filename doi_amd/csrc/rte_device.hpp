@@ -255,6 +255,38 @@ __device__ __forceinline__ void pair_axpy_rt(T (&acc)[(L + 1) * (L + 1)], T t, c
     for (int a = 1; a < (L + 1) * (L + 1); ++a) acc[a] += t * G.R[a];
 }
 
+// the two pair forms behind one interface (chosen per kernel by the geometry type)
+template <typename T, int L>
+__device__ __forceinline__ void pair_setup(PairRt<T, L> &G, T x, T y, T z, const ShConst<T> &sh)
+{
+    pair_rt<T, L>(G, x, y, z, sh);
+}
+template <typename T, int L>
+__device__ __forceinline__ void pair_setup(PairGeom<T, L> &G, T x, T y, T z, const ShConst<T> &sh)
+{
+    pair_geom<T, L>(G, x, y, z, sh);
+}
+template <typename T, int L>
+__device__ __forceinline__ T pair_proj(const T (&w)[(L + 1) * (L + 1)], const PairRt<T, L> &G)
+{
+    return pair_dot_rt<T, L>(w, G);
+}
+template <typename T, int L>
+__device__ __forceinline__ T pair_proj(const T (&w)[(L + 1) * (L + 1)], const PairGeom<T, L> &G)
+{
+    return pair_dot<T, L>(w, G);
+}
+template <typename T, int L>
+__device__ __forceinline__ void pair_back(T (&acc)[(L + 1) * (L + 1)], T t, const PairRt<T, L> &G)
+{
+    pair_axpy_rt<T, L>(acc, t, G);
+}
+template <typename T, int L>
+__device__ __forceinline__ void pair_back(T (&acc)[(L + 1) * (L + 1)], T t, const PairGeom<T, L> &G)
+{
+    pair_axpy<T, L>(acc, t, G);
+}
+
 // Walks the same Siddon crossings as march_tau and calls f(v, length) for
 // every voxel v crossed (length = r * dalpha, the destination's half step
 // included): the transpose of the path integral, used to back-project.
