@@ -986,6 +986,12 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
                     }
                 }
             }
+            // solves left alone on their medium go in pairs across media: they share
+            // the pair geometry and read their own tables (their bits do not change)
+            std::vector<int> sep;
+            const size_t npair = by_size[1].size() / 2;
+            sep.assign(by_size[1].begin(), by_size[1].begin() + 2 * npair);
+            by_size[1].erase(by_size[1].begin(), by_size[1].begin() + 2 * npair);
             svgrp.clear();
             std::vector<std::pair<int, int>> launches;  // (nr, offset)
             for (int k = kMaxNR; k >= 1; --k) {
@@ -993,6 +999,8 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
                 launches.emplace_back(k, (int)svgrp.size());
                 for (int q : by_size[k]) svgrp.push_back(svact[q]);
             }
+            const int sep_off = (int)svgrp.size();
+            for (int q : sep) svgrp.push_back(svact[q]);
             HIP_TRY(hipMemcpyAsync(svd_grp, svgrp.data(), sizeof(SolveDev<T, TA>) * svgrp.size(),
                                    hipMemcpyHostToDevice, st));
             for (auto &ln : launches) {
@@ -1000,6 +1008,11 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
                 LaunchTimer tm(ctx, 0);
                 HIP_TRY(launch_prop<T, TA>(L, path, k, pa, svd_grp + ln.second, ng, st));
                 tm.done(cp_pairs * k * ng, (flops_dir + k * flops_rhs) * ng);
+            }
+            if (npair) {
+                LaunchTimer tm(ctx, 0);
+                HIP_TRY(launch_prop_sep<T, TA>(L, 2, pa, svd_grp + sep_off, (int)npair, st));
+                tm.done(cp_pairs * 2 * npair, (flops_dir + 2 * flops_rhs) * npair);
             }
         }
         {
@@ -2268,7 +2281,13 @@ struct Bucket : BucketBase {
             if (it == graphs.end()) {
                 HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
                 const SolveDev<T, TA> *d = d_all.as<SolveDev<T, TA>>();
-                hipError_t e = launch_prop<T, TA>(L, p.path_mode, 1, pa, d, na, stream);
+                // resident solves in pairs across media on the mixed cached path (they
+                // share the pair geometry, each reads its own table), the odd one alone
+                hipError_t e = hipSuccess;
+                const int npair = (sizeof(T) == 4 && sizeof(TA) == 8 && p.path_mode == RTE_PATH_CACHED) ? na / 2 : 0;
+                if (npair) e = launch_prop_sep<T, TA>(L, 2, pa, d, npair, stream);
+                if (e == hipSuccess && na > 2 * npair)
+                    e = launch_prop<T, TA>(L, p.path_mode, 1, pa, d + 2 * npair, na - 2 * npair, stream);
                 if (e == hipSuccess) e = launch_combine<T, TA>(L, ca, d, na, stream);
                 if (e == hipSuccess) e = launch_bottom<T, TA>(P, d, na, stream);
                 hipGraph_t gr = nullptr;

@@ -105,6 +105,11 @@ hipError_t launch_fill(T *p, size_t n, T v, hipStream_t st);
 template <typename T, typename TA>
 hipError_t launch_prop(int L, int path, int nr, const PropArgs<T> &A, const SolveDev<T, TA> *sv, int ngroups,
                        hipStream_t st);
+// groups of nr (2 or 3) solves on different media of one grid (mixed precision, cached
+// path): they share the pair geometry, each reads its own path table
+template <typename T, typename TA>
+hipError_t launch_prop_sep(int L, int nr, const PropArgs<T> &A, const SolveDev<T, TA> *sv, int ngroups,
+                           hipStream_t st);
 // frequency-domain solves: group z = (sv[2z], sv[2z+1]) = (real, imaginary) half of one complex field
 template <typename T, typename TA>
 hipError_t launch_prop_c(int L, int path, const PropArgs<T> &A, const SolveDev<T, TA> *sv, int ncomplex,

@@ -38,10 +38,14 @@ __global__ void k_fill(T *__restrict__ p, size_t n, T v)
 //   and applied to every right-hand side.
 // PATH 0: kf from the per-medium table; PATH 1: marched per pair.
 // ---------------------------------------------------------------------------
-template <typename T, typename TA, int L, int PATH, int NR, bool CPLX = false>
+template <typename T, typename TA, int L, int PATH, int NR, bool CPLX = false, bool SEP = false>
 __global__ __launch_bounds__(kWave) void k_prop(PropArgs<T> A, const SolveDev<T, TA> *__restrict__ sv)
 {
     static_assert(!CPLX || NR == 2, "a complex field is propagated as its two real halves");
+    // SEP: the group's right-hand sides are solves on different media of one grid; they
+    // share the pair geometry and each reads its own path table
+    static_assert(!SEP || (PATH == 0 && !CPLX), "separate media need the cached path");
+    constexpr int NT = SEP ? NR : 1;  // path tables read per pair
     constexpr int K = (L + 1) * (L + 1);
     const GridDev<T> &g = A.g;
     const int lane = threadIdx.x, blk = blockIdx.x, chunk = blockIdx.y;
@@ -64,15 +68,21 @@ __global__ __launch_bounds__(kWave) void k_prop(PropArgs<T> A, const SolveDev<T,
 #pragma unroll
         for (int a = 0; a < K; ++a) acc[r][a] = T(0);
     }
-    const gptr<T> ktab = as_global<T>(grp[0].ktab);
+    gptr<T> ktab[NT];
+#pragma unroll
+    for (int r = 0; r < NT; ++r) ktab[r] = as_global<T>(grp[r].ktab);
     const gptr<T> mut = as_global<T>(grp[0].mut);
     const T kim = grp[0].kim;
     // destination coordinates; source coordinates advance by dx along x
     const T dxc = T(cdi) * g.dx, dyc = T(cdj) * g.dy, dzc = T(cdk) * g.dz;
     int si = s0 % g.nx, sj = (s0 / g.nx) % g.ny, sk = s0 / nxy;
-    T kf_next = T(0);
+    T kf_next[NT];
+#pragma unroll
+    for (int r = 0; r < NT; ++r) kf_next[r] = T(0);
     if constexpr (PATH == 0) {
-        if (s0 < s1) kf_next = ktab[(size_t)s0 * g.npad + slot];
+#pragma unroll
+        for (int r = 0; r < NT; ++r)
+            if (s0 < s1) kf_next[r] = ktab[r][(size_t)s0 * g.npad + slot];
     }
     T w[K];
 #pragma unroll
@@ -81,17 +91,21 @@ __global__ __launch_bounds__(kWave) void k_prop(PropArgs<T> A, const SolveDev<T,
         const T ddx = dxc - T(si) * g.dx, ddy = dyc - T(sj) * g.dy, ddz = dzc - T(sk) * g.dz;
         T r2 = ddx * ddx + ddy * ddy + ddz * ddz;
         const bool self = (cdi == si) && (cdj == sj) && (cdk == sk);
-        T kf;
+        T kfs[NT];
         if constexpr (PATH == 0) {
-            kf = kf_next;
-            if (s + 1 < s1) kf_next = ktab[(size_t)(s + 1) * g.npad + slot];
+#pragma unroll
+            for (int r = 0; r < NT; ++r) {
+                kfs[r] = kf_next[r];
+                if (s + 1 < s1) kf_next[r] = ktab[r][(size_t)(s + 1) * g.npad + slot];
+            }
         } else {
             const int ix = cdi - si, iy = cdj - sj, iz = cdk - sk;
             const bool use = valid && !self && (r2 < A.thr2);
             const T r = sqrt(self ? T(1) : r2);
             const T tau = use ? march_tau<T>(mut, s, ix, iy, iz, g.nx, nxy, r) : T(0);
-            kf = use ? dev_exp<T>(-tau) / r2 : T(0);
+            kfs[0] = use ? dev_exp<T>(-tau) / r2 : T(0);
         }
+        const T kf = kfs[0];
         r2 = self ? T(1) : r2;
         const T rinv = dev_rsqrt<T>(r2);
         // the expanded harmonics everywhere except the FP32 complex group, where
@@ -130,7 +144,7 @@ __global__ __launch_bounds__(kWave) void k_prop(PropArgs<T> A, const SolveDev<T,
             // above the axpy (two live sets of W do not fit the SGPR file).
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
-                const T t = kf * pair_proj<T, L>(w, G);
+                const T t = kfs[SEP ? r : 0] * pair_proj<T, L>(w, G);
 #pragma unroll
                 for (int a = 0; a < K; ++a)
                     w[a] = (r + 1 < NR) ? Wsrc[r + 1][(size_t)s * K + a] : Wsrc[0][(size_t)snext * K + a];
@@ -801,6 +815,25 @@ hipError_t launch_prop(int L, int path, int nr, const PropArgs<T> &A, const Solv
 }
 
 template <typename T, typename TA>
+hipError_t launch_prop_sep(int L, int nr, const PropArgs<T> &A, const SolveDev<T, TA> *sv, int ngroups,
+                           hipStream_t st)
+{
+    dim3 grid(A.g.nblk, A.nchunk, ngroups), block(kWave);
+    if constexpr (sizeof(T) == 4 && sizeof(TA) == 8) {
+        if (nr != 2 && nr != 3) return hipErrorInvalidValue;
+        RTE_DISPATCH_L(L, {
+            if (nr == 2)
+                hipLaunchKernelGGL((k_prop<T, TA, LL, 0, 2, false, true>), grid, block, 0, st, A, sv);
+            else
+                hipLaunchKernelGGL((k_prop<T, TA, LL, 0, 3, false, true>), grid, block, 0, st, A, sv);
+        });
+        return hipGetLastError();
+    } else {
+        return hipErrorInvalidValue;
+    }
+}
+
+template <typename T, typename TA>
 hipError_t launch_prop_c(int L, int path, const PropArgs<T> &A, const SolveDev<T, TA> *sv, int ncomplex,
                          hipStream_t st)
 {
@@ -941,6 +974,12 @@ hipError_t launch_tau_list(const GridDev<float> &g, const float *mut, int np, co
 }
 
 // explicit instantiations used by the host library
+template hipError_t launch_prop_sep<float, double>(int, int, const PropArgs<float> &, const SolveDev<float, double> *,
+                                                   int, hipStream_t);
+template hipError_t launch_prop_sep<double, double>(int, int, const PropArgs<double> &,
+                                                    const SolveDev<double, double> *, int, hipStream_t);
+template hipError_t launch_prop_sep<float, float>(int, int, const PropArgs<float> &, const SolveDev<float, float> *,
+                                                  int, hipStream_t);
 template hipError_t launch_prop_c<float, double>(int, int, const PropArgs<float> &, const SolveDev<float, double> *, int,
                                                      hipStream_t);
 template hipError_t launch_combine_c<float, double>(int, const CombArgs<float, double> &, const SolveDev<float, double> *, int,
