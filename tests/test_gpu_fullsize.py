@@ -71,3 +71,30 @@ def test_32cube_symmetry_linearity_batch(ctx):
     assert np.array_equal(alone[0], imgs[2])
     m.close()
     other.close()
+
+
+def test_32cube_frequency_domain_properties(ctx):
+    """The frequency-domain solve at 32^3, L = 5 (100 MHz): the mixed path within
+    1e-4 of the FP64 one, -omega the complex conjugate, the modulus linear in the
+    beam power and omega -> 0 the continuous-wave image."""
+    n, L, om = 32, 5, 2.0 * np.pi * 1.0e8
+    ma, ms = media.lognormal_media_with_inclusions(1, seed=5, n=n)[0]
+    m = ctx.medium((n, n, n), media.cube_bounds(n), ma, ms, 0.9)
+    b = media.top_face_beams(seed=5, n=n)[0]
+    out = {}
+    for prec in (rte.PREC_FP64, rte.PREC_MIXED):
+        print("32^3 fd", prec, flush=True)
+        p = rte.make_params(L, precision=prec, stop_rule=rte.STOP_DELTA, tol=1e-6, max_terms=12)
+        out[prec] = ctx.solve_fd([m], [b], [om], p)
+    e64, e32 = out[rte.PREC_FP64][0][0], out[rte.PREC_MIXED][0][0]
+    assert out[rte.PREC_MIXED][2][0]["terms"] == out[rte.PREC_FP64][2][0]["terms"]
+    assert np.max(np.abs(e32 - e64)) < 1e-4 * np.max(np.abs(e64))
+    p = rte.make_params(L, precision=rte.PREC_MIXED, stop_rule=rte.STOP_DELTA, tol=1e-6, max_terms=12)
+    neg, _, _ = ctx.solve_fd([m], [b], [-om], p)
+    assert np.max(np.abs(neg[0] - np.conj(e32))) <= 1e-14 * np.max(np.abs(e32))
+    twice, _, _ = ctx.solve_fd([m], [(b[0], b[1], b[2], 2.0)], [om], p)
+    assert np.max(np.abs(twice[0] - 2.0 * e32)) < 1e-6 * np.max(np.abs(e32))
+    cw, _ = ctx.solve_media([m], [b], p)
+    slow, _, _ = ctx.solve_fd([m], [b], [1.0], p)       # 1 rad/s: phase ~1e-10
+    assert np.max(np.abs(slow[0] - cw[0])) < 1e-6 * np.max(np.abs(cw[0]))
+    m.close()
