@@ -81,7 +81,7 @@ __global__ __launch_bounds__(kWave) void k_prop_adj(PropArgs<T> A, AdjDev<T, TA>
         r2 = self ? T(1) : r2;
         const T rinv = dev_rsqrt<T>(r2);
         PairGeom<T, L> G;
-        pair_geom<T, L>(G, ddx * rinv, ddy * rinv, ddz * rinv, A.sh);
+        pair_geom<T, L>(G, ddx * rinv, ddy * rinv, ddz * rinv);
         const T t = kf * pair_dot<T, L>(w, G);
         pair_axpy<T, L>(acc, t, G);
         if (valid) {

@@ -25,15 +25,49 @@ __device__ __forceinline__ float dev_exp<float>(float x) { return expf(x); }
 template <>
 __device__ __forceinline__ double dev_exp<double>(double x) { return exp(x); }
 
-// sin and cos of the pair phase k r (|k r| of order 10 at most): in FP32 the
-// hardware sine/cosine (absolute error ~1e-6 at these arguments, below the
-// FP32 path's tolerance), in FP64 the accurate library routine
+// FP64 sine and cosine with no branch, for |x| < kSinCosMax.  The library
+// routine switches per lane to a large-argument reduction; inside the
+// propagation loop that divergent branch is where the register allocator has
+// placed spill copies of values live in every lane under the branch's partial
+// EXEC mask, losing them in the other lanes.  Here: Cody-Waite reduction
+// x = k pi/2 + r with pi/2 = P1 + P2 + P3 (P1, P2 of 31 and 32 significant
+// bits, so k P1 and k P2 are exact for |k| < 2^21, and x - k P1 is exact),
+// then the Taylor series of sin and cos on |r| <= pi/4 to r^19 and r^20
+// (next terms below 2^-60 relative) and the quadrant by selects.  Absolute
+// error below 2^-52 on |x| < kSinCosMax; sin(-x) = -sin x, cos(-x) = cos x exactly.
+__host__ __device__ constexpr double factorial(int n) { return n <= 1 ? 1.0 : n * factorial(n - 1); }  // exact to 22!
+__host__ __device__ constexpr double inv_factorial(int n) { return 1.0 / factorial(n); }
+__host__ __device__ __forceinline__ void sincos_cw(double x, double *s, double *c)
+{
+    constexpr double P1 = 0x1.921fb544p+0, P2 = 0x1.0b4611a6p-34, P3 = 0x1.3198a2e037073p-69;
+    constexpr double TWO_OVER_PI = 0x1.45f306dc9c883p-1;
+    const double k = rint(x * TWO_OVER_PI);
+    double r = fma(-k, P1, x);
+    r = fma(-k, P2, r);
+    r = fma(-k, P3, r);
+    const double r2 = r * r;
+    // sin r = r + r^3 S(r^2), cos r = 1 + r^2 C(r^2)
+    double S = -inv_factorial(19), C = inv_factorial(20);
+#pragma unroll
+    for (int j = 8; j >= 1; --j) S = fma(S, r2, (j & 1) ? -inv_factorial(2 * j + 1) : inv_factorial(2 * j + 1));
+#pragma unroll
+    for (int j = 9; j >= 1; --j) C = fma(C, r2, (j & 1) ? -inv_factorial(2 * j) : inv_factorial(2 * j));
+    const double sr = fma(r * r2, S, r), cr = fma(r2, C, 1.0);
+    const int q = (int)k & 3;
+    const double a = (q & 1) ? cr : sr, b = (q & 1) ? sr : cr;
+    *s = (q & 2) ? -a : a;
+    *c = ((q + 1) & 2) ? -b : b;
+}
+
+// sin and cos of the pair phase k r: in FP32 the hardware sine/cosine
+// (absolute error ~1e-6 at arguments of order 10, below the FP32 path's
+// tolerance), in FP64 the branch-free routine above
 template <typename T>
 __device__ __forceinline__ void dev_sincos(T x, T *s, T *c);
 template <>
 __device__ __forceinline__ void dev_sincos<float>(float x, float *s, float *c) { __sincosf(x, s, c); }
 template <>
-__device__ __forceinline__ void dev_sincos<double>(double x, double *s, double *c) { sincos(x, s, c); }
+__device__ __forceinline__ void dev_sincos<double>(double x, double *s, double *c) { sincos_cw(x, s, c); }
 
 template <typename T>
 __device__ __forceinline__ T big_value();
@@ -86,7 +120,7 @@ __device__ __forceinline__ T march_tau(P mut, int v, int ix, int iy, int iz, int
 
 // ---------------------------------------------------------------------------
 // One (destination, source) pair in the factored real-SH form, on scaled
-// coefficients (see ShConst):  p = sum_a Rt_a(u) w'_a ;  acc' += (kf p) Rt(u)
+// coefficients (see sh_gamma):  p = sum_a Rt_a(u) w'_a ;  acc' += (kf p) Rt(u)
 // with Rt_l0 = Pt_l^0, Rt_lm = Pt_l^m C_m, Rt_l,-m = Pt_l^m S_m,
 // C_m + i S_m = (x + i y)^m.  The direction part (pair_geom) is shared by all
 // right-hand sides that use the same pair (beams on one medium).
@@ -98,7 +132,7 @@ struct PairGeom {
 };
 
 template <typename T, int L>
-__device__ __forceinline__ void pair_geom(PairGeom<T, L> &G, T x, T y, T z, const ShConst<T> &sh)
+__device__ __forceinline__ void pair_geom(PairGeom<T, L> &G, T x, T y, T z)
 {
     G.C[0] = T(1);
     G.S[0] = T(0);
@@ -116,7 +150,7 @@ __device__ __forceinline__ void pair_geom(PairGeom<T, L> &G, T x, T y, T z, cons
         G.P[m][m] = T(1);
         if (m + 1 <= L) G.P[m + 1][m] = z;
 #pragma unroll
-        for (int l = m + 2; l <= L; ++l) G.P[l][m] = z * G.P[l - 1][m] - sh.gamma[l][m] * G.P[l - 2][m];
+        for (int l = m + 2; l <= L; ++l) G.P[l][m] = z * G.P[l - 1][m] - sh_gamma<T>(l, m) * G.P[l - 2][m];
     }
 }
 
@@ -185,14 +219,14 @@ struct PairRt {
 };
 
 template <typename T, int L>
-__device__ __forceinline__ void pair_rt(PairRt<T, L> &G, T x, T y, T z, const ShConst<T> &sh)
+__device__ __forceinline__ void pair_rt(PairRt<T, L> &G, T x, T y, T z)
 {
     if constexpr (L >= 1) {
         G.R[2] = z;
         T p2 = T(1), p1 = z;
 #pragma unroll
         for (int l = 2; l <= L; ++l) {
-            const T p = z * p1 - sh.gamma[l][0] * p2;
+            const T p = z * p1 - sh_gamma<T>(l, 0) * p2;
             G.R[l * l + l] = p;
             p2 = p1;
             p1 = p;
@@ -213,7 +247,7 @@ __device__ __forceinline__ void pair_rt(PairRt<T, L> &G, T x, T y, T z, const Sh
                 G.R[(m + 1) * (m + 1) + (m + 1) - m] = s1;
 #pragma unroll
                 for (int l = m + 2; l <= L; ++l) {
-                    const T cl = z * c1 - sh.gamma[l][m] * c2, sl = z * s1 - sh.gamma[l][m] * s2;
+                    const T cl = z * c1 - sh_gamma<T>(l, m) * c2, sl = z * s1 - sh_gamma<T>(l, m) * s2;
                     G.R[l * l + l + m] = cl;
                     G.R[l * l + l - m] = sl;
                     c2 = c1;
@@ -257,14 +291,14 @@ __device__ __forceinline__ void pair_axpy_rt(T (&acc)[(L + 1) * (L + 1)], T t, c
 
 // the two pair forms behind one interface (chosen per kernel by the geometry type)
 template <typename T, int L>
-__device__ __forceinline__ void pair_setup(PairRt<T, L> &G, T x, T y, T z, const ShConst<T> &sh)
+__device__ __forceinline__ void pair_setup(PairRt<T, L> &G, T x, T y, T z)
 {
-    pair_rt<T, L>(G, x, y, z, sh);
+    pair_rt<T, L>(G, x, y, z);
 }
 template <typename T, int L>
-__device__ __forceinline__ void pair_setup(PairGeom<T, L> &G, T x, T y, T z, const ShConst<T> &sh)
+__device__ __forceinline__ void pair_setup(PairGeom<T, L> &G, T x, T y, T z)
 {
-    pair_geom<T, L>(G, x, y, z, sh);
+    pair_geom<T, L>(G, x, y, z);
 }
 template <typename T, int L>
 __device__ __forceinline__ T pair_proj(const T (&w)[(L + 1) * (L + 1)], const PairRt<T, L> &G)

@@ -96,7 +96,7 @@ static double nlm(int l, int m)
     return std::sqrt((2.0 * l + 1.0) / (4.0 * M_PI * f));
 }
 
-// n_a of the scaled coefficients and gamma of the monic recurrences (see ShConst)
+// n_a of the scaled coefficients (the monic recurrences' gamma is sh_gamma)
 static double sh_scale(int l, int m)
 {
     double c = ((m & 1) ? -1.0 : 1.0);
@@ -113,21 +113,9 @@ static double sh_scale_a(int a)
     return sh_scale(l, m < 0 ? -m : m);
 }
 
-template <typename T>
-static ShConst<T> make_sh_const()
-{
-    ShConst<T> c;
-    std::memset(&c, 0, sizeof c);
-    for (int m = 0; m <= kMaxL; ++m)
-        for (int l = m + 2; l <= kMaxL; ++l)
-            c.gamma[l][m] = T((double)(l + m - 1) * (double)(l - m - 1) / ((2.0 * l - 1.0) * (2.0 * l - 3.0)));
-    return c;
-}
-
 // real SH R_a(x,y,z) for a unit vector, with the recurrences of the kernels
 static void real_sh(int L, double x, double y, double z, double *out)
 {
-    static const ShConst<double> sh = make_sh_const<double>();
     double C[kMaxL + 1], S[kMaxL + 1], P[kMaxL + 1][kMaxL + 1];
     C[0] = 1.0;
     S[0] = 0.0;
@@ -138,7 +126,7 @@ static void real_sh(int L, double x, double y, double z, double *out)
     for (int m = 0; m <= L; ++m) {
         P[m][m] = 1.0;
         if (m + 1 <= L) P[m + 1][m] = z;
-        for (int l = m + 2; l <= L; ++l) P[l][m] = z * P[l - 1][m] - sh.gamma[l][m] * P[l - 2][m];
+        for (int l = m + 2; l <= L; ++l) P[l][m] = z * P[l - 1][m] - sh_gamma<double>(l, m) * P[l - 2][m];
     }
     for (int l = 0; l <= L; ++l) {
         out[l * l + l] = sh_scale(l, 0) * P[l][0];
@@ -916,7 +904,6 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
 
     PropArgs<T> pa;
     pa.g = g;
-    pa.sh = make_sh_const<T>();
     pa.nchunk = nch;
     pa.chunk_len = clen;
     pa.thr2 = thr > 0 ? T(thr * thr) : T(sizeof(T) == 4 ? 1e30 : 1e300);
@@ -1179,6 +1166,12 @@ extern "C" int rte_solve_fd(rte_ctx *ctx, int nb, rte_medium *const *media, cons
         if (!std::isfinite(omega[i])) return fail(RTE_E_ARG, "omega[%d] is not finite", i);
         // CPU package/src/RTE_uniform.c:126: mu_a + i modulation / (c / n)
         kim[i] = omega[i] * media[i]->n / media[i]->c;
+        // the phase e^{-i kim r} of the farthest voxel pair must stay in the sine's range
+        const double ex = (a.xmax - a.xmin) * (a.nx - 1) / a.nx, ey = (a.ymax - a.ymin) * (a.ny - 1) / a.ny,
+                     ez = (a.zmax - a.zmin) * (a.nz - 1) / a.nz;
+        if (std::fabs(kim[i]) * std::sqrt(ex * ex + ey * ey + ez * ez) >= kSinCosMax)
+            return fail(RTE_E_ARG, "omega[%d]: phase omega n / c times the grid diagonal is %g rad, above %g", i,
+                        std::fabs(kim[i]) * std::sqrt(ex * ex + ey * ey + ez * ez), kSinCosMax);
     }
     if ((rc = set_device(ctx))) return rc;
     switch (params->precision) {
@@ -1326,7 +1319,6 @@ struct AdjointRun {
             h.state = (max_terms <= 1) ? 2 : 0;
         }
         pa.g = g;
-        pa.sh = make_sh_const<T>();
         pa.nchunk = nch;
         pa.chunk_len = clen;
         pa.thr2 = thr > 0 ? T(thr * thr) : T(sizeof(T) == 4 ? 1e30 : 1e300);
@@ -1786,7 +1778,6 @@ static int propagate_impl(rte_ctx *ctx, rte_medium *m, int L, double prop_thresh
     HIP_TRY(hipMemcpyAsync(dSv.p, &s, sizeof s, hipMemcpyHostToDevice, st));
     PropArgs<T> pa;
     pa.g = g;
-    pa.sh = make_sh_const<T>();
     pa.nchunk = nch;
     pa.chunk_len = clen;
     pa.thr2 = thr > 0 ? T(thr * thr) : T(sizeof(T) == 4 ? 1e30 : 1e300);
@@ -2125,7 +2116,6 @@ struct Bucket : BucketBase {
         nch = (N + clen - 1) / clen;
         sc = make_scales(L);
         pa.g = g;
-        pa.sh = make_sh_const<T>();
         pa.nchunk = nch;
         pa.chunk_len = clen;
         pa.thr2 = thr > 0 ? T(thr * thr) : T(sizeof(T) == 4 ? 1e30 : 1e300);

@@ -15,6 +15,8 @@ constexpr int kNOmega = kAngRes * kAngRes;
 constexpr int kBrick = 4;      // destination bricks of 4x4x4 voxels = one wave64
 constexpr int kWave = 64;
 constexpr int kMaxNR = 5;      // right-hand sides (beams on one medium) per propagation wave
+// largest pair phase |omega n / c| r of a frequency-domain solve (FP64 sine/cosine range)
+constexpr double kSinCosMax = 1.0e6;
 
 // Geometry in device form.  Voxel v = x + nx*(y + ny*z); destination bricks
 // are numbered bx fastest; slot = brick*64 + lane, lane = lx + 4*(ly + 4*lz).
@@ -34,10 +36,13 @@ struct GridDev {
 // R_l,-m = n_lm Pt_l^m Im((x+iy)^m), n_lm = s_m N_lm c_lm (s_m = sqrt2 for m>0).
 // Device fields hold the scaled coefficients W'_a = n_a W_a, so a pair costs
 // no multiplication by n; partial sums come out scaled by 1/n_a.
+// gamma_lm is a compile-time constant (rounded once from double): in the
+// unrolled recurrences it folds into the instructions and takes no registers.
 template <typename T>
-struct ShConst {
-    T gamma[kMaxL + 1][kMaxL + 1];
-};
+__host__ __device__ constexpr T sh_gamma(int l, int m)
+{
+    return T(double(l + m - 1) * double(l - m - 1) / ((2.0 * l - 1.0) * (2.0 * l - 3.0)));
+}
 
 // Per-solve device state of one Neumann series.
 template <typename T, typename TA>
@@ -63,7 +68,6 @@ struct SolveDev {
 template <typename T>
 struct PropArgs {
     GridDev<T> g;
-    ShConst<T> sh;
     int nchunk, chunk_len;
     T thr2;  // prop_thresh^2; pairs with r^2 >= thr2 are dropped
 };

@@ -112,7 +112,7 @@ __global__ __launch_bounds__(kWave) void k_prop(PropArgs<T> A, const SolveDev<T,
         // their extra registers cost more than the per-side products they save
         using Geom = typename std::conditional<CPLX && sizeof(T) == 4, PairGeom<T, L>, PairRt<T, L>>::type;
         Geom G;
-        pair_setup<T, L>(G, ddx * rinv, ddy * rinv, ddz * rinv, A.sh);
+        pair_setup<T, L>(G, ddx * rinv, ddy * rinv, ddz * rinv);
         // w holds right-hand side r of source s; the next one (r+1, or 0 of
         // s+1) is requested before r is applied, hiding the scalar-load latency
         const int snext = (s + 1 < s1) ? s + 1 : s;
