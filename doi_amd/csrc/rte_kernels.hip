@@ -136,6 +136,31 @@ __global__ __launch_bounds__(kWave) void k_prop(PropArgs<T> A, const SolveDev<T,
             pair_back<T, L>(acc[0], kc * p0 + ks * p1, G);
             pair_back<T, L>(acc[1], kc * p1 - ks * p0, G);
             __builtin_amdgcn_sched_barrier(0);
+        } else if constexpr (sizeof(T) == 4 && K <= 36 && NR == 2) {
+            // Groups of two: two sets of scalar registers hold W (2 x 36 floats
+            // fit beside the loop's other scalars; with three right-hand sides
+            // they do not, and the spills cost more than the latency saved).
+            // The next right-hand side's row, or the next source's first, is
+            // requested before this side's projection, so a projection and an
+            // axpy cover each scalar load (74 and 151 VALU instructions, against
+            // 53 and 70).  Scalar loads return out of order, so a wait is always
+            // for all of them: this side's row is waited for first (lgkmcnt(0);
+            // vmcnt and expcnt left at their maxima), then the next requested.
+            T wn[K];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                __builtin_amdgcn_s_waitcnt(0xC07F);
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int a = 0; a < K; ++a)
+                    wn[a] = (r + 1 < NR) ? Wsrc[r + 1][(size_t)s * K + a] : Wsrc[0][(size_t)snext * K + a];
+                __builtin_amdgcn_sched_barrier(0);
+                const T t = kfs[SEP ? r : 0] * pair_proj<T, L>(w, G);
+                pair_back<T, L>(acc[r], t, G);
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int a = 0; a < K; ++a) w[a] = wn[a];
+            }
         } else {
             // One set of scalar registers holds W: the projection reads it, the
             // next right-hand side (or source) is then loaded into the same
