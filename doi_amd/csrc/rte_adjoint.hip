@@ -379,6 +379,8 @@ __global__ void k_src_grad(int nx, int ny, int nz, double dxd, double dyd, doubl
 // region of the forward field phi = sum_q A W_q times -1 (mu_a) or g^l - 1
 // (mu_s), in scaled coefficients; its pixel buffers start at zero (no
 // ballistic term) with the bottom-layer stop-rule sums of that first term.
+// wrt 2 with part == nullptr: phi itself, every voxel (a given first term,
+// the reference's Neumann(src, out, 1), FisherInfo_IO/src/neumann.cu:23).
 // grid = (ceil(N/64), nsolve), block = 64.
 // ---------------------------------------------------------------------------
 template <typename T, typename TA, int L>
@@ -391,11 +393,12 @@ __global__ __launch_bounds__(kWave) void k_ref_source(GridDev<T> g, Scales sc, c
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= g.N) return;
     const SolveDev<T, TA> S = sv[blockIdx.y];
-    const bool in = part[v] == param[blockIdx.y];
+    // part == nullptr: every voxel (a given first term, wrt 2)
+    const bool in = part ? part[v] == param[blockIdx.y] : true;
     TA gl = TA(1), bs = TA(0);
 #pragma unroll
     for (int l = 0; l <= L; ++l) {
-        const TA f = wrt == 0 ? TA(-1) : gl - TA(1);
+        const TA f = wrt == 0 ? TA(-1) : wrt == 1 ? gl - TA(1) : TA(1);
 #pragma unroll
         for (int m = -l; m <= l; ++m) {
             const int a = l * l + l + m;

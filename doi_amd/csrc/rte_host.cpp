@@ -1145,6 +1145,64 @@ extern "C" int rte_solve_field(rte_ctx *ctx, rte_medium *m, const rte_beam *beam
     }
 }
 
+// The series from a given first term (FisherInfo_IO/src/neumann.cu:23-120, flag 1):
+// the source enters as the derivative solves' first term does (k_ref_source, wrt 2:
+// factor 1 on every voxel); the first term is the same in the reference's units and
+// this library's (A carries 1/c_m there and D carries c_m), so it is uploaded as given.
+template <typename T, typename TA>
+static int neumann_run(rte_ctx *ctx, rte_medium *m, const rte_params *p, const double *src, double *field,
+                       double *exitance, double *reflectance, rte_stats *stats)
+{
+    const int L = p->L, K = (L + 1) * (L + 1), N = m->N, P = m->grid.nx * m->grid.ny;
+    hipStream_t st = ctx->stream;
+    std::vector<TA> vm((size_t)N * K);  // voxel-major, as k_ref_source reads phi
+    for (int a = 0; a < K; ++a)
+        for (int v = 0; v < N; ++v) vm[(size_t)v * K + a] = TA(src[(size_t)a * N + v]);
+    std::vector<TA> ws(K);
+    for (int a = 0; a < K; ++a) ws[a] = TA(stop_weight(a));
+    const int param0 = 0;
+    DevBuf dSrc, dParam, dWs;
+    HIP_TRY(dSrc.alloc(sizeof(TA) * vm.size()));
+    HIP_TRY(dParam.alloc(sizeof(int)));
+    HIP_TRY(dWs.alloc(sizeof(TA) * K));
+    HIP_TRY(hipMemcpyAsync(dSrc.p, vm.data(), sizeof(TA) * vm.size(), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(dParam.p, &param0, sizeof(int), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(dWs.p, ws.data(), sizeof(TA) * K, hipMemcpyHostToDevice, st));
+    RefSrc<TA> rs;
+    rs.part = nullptr;
+    rs.param = dParam.as<int>();
+    rs.wrt = 2;
+    rs.phi = dSrc.as<TA>();
+    rs.wsum = dWs.as<TA>();
+    std::vector<double> img;
+    if (!exitance) {
+        img.resize(P);
+        exitance = img.data();
+    }
+    rte_medium *const media[1] = {m};
+    int rc = run_batch<T, TA>(ctx, 1, media, nullptr, p, exitance, stats, &rs, reflectance, field);
+    HIP_TRY(hipStreamSynchronize(st));  // the uploads' host buffers end here
+    return rc;
+}
+
+extern "C" int rte_neumann(rte_ctx *ctx, rte_medium *m, const rte_params *params, const double *src,
+                           double *field, double *exitance, double *reflectance, rte_stats *stats)
+{
+    if (!ctx || !m || !src || !field) return fail(RTE_E_ARG, "rte_neumann: NULL argument");
+    if (m->ctx != ctx) return fail(RTE_E_ARG, "rte_neumann: medium belongs to another context");
+    int rc;
+    if ((rc = check_params(params))) return rc;
+    if (params->precision != RTE_PREC_FP64 && params->precision != RTE_PREC_MIXED)
+        return fail(RTE_E_UNSUPP, "rte_neumann: precision must be RTE_PREC_FP64 or RTE_PREC_MIXED");
+    const size_t n = (size_t)(params->L + 1) * (params->L + 1) * m->N;
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(src[i])) return fail(RTE_E_ARG, "rte_neumann: src[%zu] is not finite", i);
+    if ((rc = set_device(ctx))) return rc;
+    if (params->precision == RTE_PREC_FP64)
+        return neumann_run<double, double>(ctx, m, params, src, field, exitance, reflectance, stats);
+    return neumann_run<float, double>(ctx, m, params, src, field, exitance, reflectance, stats);
+}
+
 extern "C" int rte_solve_fd(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_beam *beams,
                             const double *omega, const rte_params *params, double *exitance, double *reflectance,
                             double *field, rte_stats *stats)

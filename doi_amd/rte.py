@@ -105,6 +105,7 @@ SIGNATURES = {
     "rte_solve_images": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(Beam), C.POINTER(Params),
                                    _dp, _dp, C.POINTER(Stats)]),
     "rte_solve_field": (C.c_int, [_vp, _vp, C.POINTER(Beam), C.POINTER(Params), _dp, _dp, _dp, C.POINTER(Stats)]),
+    "rte_neumann": (C.c_int, [_vp, _vp, C.POINTER(Params), _dp, _dp, _dp, _dp, C.POINTER(Stats)]),
     "rte_solve_fd": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(Beam), _dp, C.POINTER(Params), _dp, _dp, _dp,
                                C.POINTER(Stats)]),
     "rte_exitance_vjp": (C.c_int, [_vp, _vp, C.POINTER(Beam), C.POINTER(Params), C.POINTER(GradOpts), _dp, _dp,
@@ -272,6 +273,21 @@ class Context:
         st = Stats()
         _check(lib().rte_solve_field(self.h, medium.h, C.byref(Beam(*beam)), C.byref(params), _p(field), _p(img),
                                      _p(refl), C.byref(st)))
+        return field.reshape(K, medium.N), img.reshape(ny, nx), refl.reshape(ny, nx), _stats_dict(st)
+
+    def neumann(self, medium, src, params):
+        """The reference's Neumann(src, out, 1) (rte_neumann): W_0 = src ([K, N], the
+        reference's units), field = A sum_q W_q.  Returns (field [K, N], transmitted
+        image, reflectance, stats), the images without a ballistic term."""
+        nx, ny, _ = medium.shape
+        K = (params.L + 1) ** 2
+        src = _f64(np.reshape(src, (K * medium.N,)))
+        field = np.zeros(K * medium.N)
+        img = np.zeros(nx * ny)
+        refl = np.zeros(nx * ny)
+        st = Stats()
+        _check(lib().rte_neumann(self.h, medium.h, C.byref(params), _p(src), _p(field), _p(img), _p(refl),
+                                 C.byref(st)))
         return field.reshape(K, medium.N), img.reshape(ny, nx), refl.reshape(ny, nx), _stats_dict(st)
 
     def solve_fd(self, media, beams, omegas, params, field=False):

@@ -179,6 +179,17 @@ int rte_solve_images(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
 int rte_solve_field(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte_params *params,
                     double *field, double *exitance, double *reflectance, rte_stats *stats);
 
+/* The reference's Neumann(src, out, 1) on a given first term, called with out
+ * preset to src as its mains do (FisherInfo_IO/src/neumann.cu:23-120,
+ * rte_fim.cu:235-237, 275-276; flag 0, out = A out, is rte_propagate): W_0 = src,
+ * W_{q+1} = D A W_q until the stop rule, field = A sum_q W_q.  src and field
+ * are K*N coefficient-major in the reference's units (rte_source's s0 is such a
+ * first term, rte_solve_field's field such a result); exitance and reflectance
+ * (nx*ny, may be NULL) are the face images of the field, with no ballistic
+ * term.  RTE_PREC_FP64 or RTE_PREC_MIXED (FP64 running sums). */
+int rte_neumann(rte_ctx *ctx, rte_medium *m, const rte_params *params, const double *src, double *field,
+                double *exitance, double *reflectance, rte_stats *stats);
+
 /* Frequency-domain solves of intensity-modulated beams: omega[i] (rad/s) is
  * the reference's [Source] modulation, which turns mu_a into
  * mu_a + i omega / (c/n) (CPU package/src/RTE_uniform.c:120-177, Inits.c:60).

@@ -78,6 +78,8 @@ def lib():
         L.orc_quad_weights_refl.argtypes = [C.c_int, C.c_int, dp]
         L.orc_ref_derivative.argtypes = [C.POINTER(Grid), C.POINTER(Optics), C.POINTER(Beam), C.POINTER(Params),
                                          C.POINTER(C.c_int), C.c_int, dp, C.POINTER(Stats)]
+        L.orc_neumann.argtypes = [C.POINTER(Grid), C.POINTER(Optics), C.POINTER(Params), dp, dp, dp, dp,
+                                  C.POINTER(Stats)]
         L.orc_bottom_sum.argtypes = [C.POINTER(Grid), C.c_int, dp]
         L.orc_bottom_sum.restype = C.c_double
         L.orc_self_block_c.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
@@ -164,6 +166,24 @@ class Problem:
         if rc != 0:
             raise RuntimeError("orc_ref_derivative failed: %d" % rc)
         return img.reshape(ny, nx), st.terms
+
+    def neumann(self, src, L, stop_rule=STOP_LAMBDA, tol=1e-6, max_terms=1000, prop_thresh=1e30,
+                exit_quad=QUAD_REF):
+        """Neumann(src, out, 1) on a given first term src [K, N]: (field A sum_q W_q
+        [K, N], exitance, reflectance, terms), the images without a ballistic term."""
+        nx, ny, _ = self.shape
+        K = (L + 1) ** 2
+        src = _f64(np.reshape(src, (K * self.N,)))
+        w = np.zeros(K * self.N)
+        img = np.zeros(nx * ny)
+        refl = np.zeros(nx * ny)
+        st = Stats()
+        prm = Params(L, max_terms, stop_rule, tol, prop_thresh, exit_quad)
+        rc = lib().orc_neumann(C.byref(self.grid), C.byref(self.optics), C.byref(prm), _p(src), _p(w), _p(img),
+                               _p(refl), C.byref(st))
+        if rc != 0:
+            raise RuntimeError("orc_neumann failed: %d" % rc)
+        return w.reshape(K, self.N), img.reshape(ny, nx), refl.reshape(ny, nx), st.terms
 
     def source(self, beam, L):
         K = (L + 1) ** 2

@@ -796,6 +796,15 @@ int orc_solve_field(const orc_grid *g, const orc_optics *o, const orc_beam *b, c
     return solve_impl(g, o, b, p, exitance, w_out, NULL, st);
 }
 
+/* FisherInfo_IO/src/neumann.cu:23-120 with flag 1, called with out preset to src
+ * (FisherInfo_IO/src/rte_fim.cu:235-237, 275-276): the series of neumann_from
+ * from the given first term, no ballistic term */
+int orc_neumann(const orc_grid *g, const orc_optics *o, const orc_params *p, const double *src,
+                double *w_out, double *exitance, double *refl, orc_stats *st)
+{
+    return neumann_from(g, o, p, src, NULL, exitance, w_out, refl, st);
+}
+
 /* ---------------------------------------------------------------------------
  * Frequency domain: complex mu_t = mu_a + mu_s + i kim, kim = omega n / c
  * (CPU package/src/RTE_uniform.c:126).  Fields are complex (interleaved

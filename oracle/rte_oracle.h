@@ -117,6 +117,12 @@ int orc_solve_field(const orc_grid *g, const orc_optics *o, const orc_beam *b,
 int orc_ref_derivative(const orc_grid *g, const orc_optics *o, const orc_beam *b, const orc_params *p,
                        const int *region, int which, double *dimg, orc_stats *st);
 
+/* the reference's Neumann(src, out, 1) on a given first term src (K*N):
+ * w_out = A sum_q (DA)^q src; exitance (nx*ny) and refl (may be NULL) are its
+ * face images, with no ballistic term */
+int orc_neumann(const orc_grid *g, const orc_optics *o, const orc_params *p, const double *src,
+                double *w_out, double *exitance, double *refl, orc_stats *st);
+
 /* Frequency domain (complex mu_t = mu_a + mu_s + i kim, kim = omega n / c).
  * Complex arrays are interleaved (re, im) doubles, fields coefficient-major. */
 int orc_self_block_c(int L, double dx, double dy, double dz, double mu_re, double mu_im, double c_m,
