@@ -122,8 +122,8 @@ __device__ __forceinline__ T march_tau(P mut, int v, int ix, int iy, int iz, int
 // One (destination, source) pair in the factored real-SH form, on scaled
 // coefficients (see sh_gamma):  p = sum_a Rt_a(u) w'_a ;  acc' += (kf p) Rt(u)
 // with Rt_l0 = Pt_l^0, Rt_lm = Pt_l^m C_m, Rt_l,-m = Pt_l^m S_m,
-// C_m + i S_m = (x + i y)^m.  The direction part (pair_geom) is shared by all
-// right-hand sides that use the same pair (beams on one medium).
+// C_m + i S_m = (x + i y)^m.  The adjoint propagation (k_prop_adj) uses this
+// form; the forward propagation uses the expanded one below.
 // ---------------------------------------------------------------------------
 template <typename T, int L>
 struct PairGeom {
@@ -196,16 +196,10 @@ __device__ __forceinline__ void pair_axpy(T (&acc)[(L + 1) * (L + 1)], T t, cons
     }
 }
 
-template <typename T, int L>
-__device__ __forceinline__ void pair_apply(T (&acc)[(L + 1) * (L + 1)], const T (&w)[(L + 1) * (L + 1)], T kf,
-                                           const PairGeom<T, L> &G)
-{
-    pair_axpy<T, L>(acc, kf * pair_dot<T, L>(w, G), G);
-}
-
 // ---------------------------------------------------------------------------
-// The same pair in expanded form for the propagation kernel: the direction's
-// scaled real harmonics Rt_a(u) are formed once per pair,
+// The same pair in expanded form for the propagation kernel (every group,
+// real or complex): the direction's scaled real harmonics Rt_a(u) are formed
+// once per pair,
 //   Rt_l0 = Pt_l^0,  Rt_l,m = C_m Pt_l^m,  Rt_l,-m = S_m Pt_l^m   (Rt_00 = 1),
 // the products by the Legendre recurrence in l itself (it is linear, so it
 // holds for C_m Pt_l^m with the same gamma):
@@ -287,38 +281,6 @@ __device__ __forceinline__ void pair_axpy_rt(T (&acc)[(L + 1) * (L + 1)], T t, c
     acc[0] += t;
 #pragma unroll
     for (int a = 1; a < (L + 1) * (L + 1); ++a) acc[a] += t * G.R[a];
-}
-
-// the two pair forms behind one interface (chosen per kernel by the geometry type)
-template <typename T, int L>
-__device__ __forceinline__ void pair_setup(PairRt<T, L> &G, T x, T y, T z)
-{
-    pair_rt<T, L>(G, x, y, z);
-}
-template <typename T, int L>
-__device__ __forceinline__ void pair_setup(PairGeom<T, L> &G, T x, T y, T z)
-{
-    pair_geom<T, L>(G, x, y, z);
-}
-template <typename T, int L>
-__device__ __forceinline__ T pair_proj(const T (&w)[(L + 1) * (L + 1)], const PairRt<T, L> &G)
-{
-    return pair_dot_rt<T, L>(w, G);
-}
-template <typename T, int L>
-__device__ __forceinline__ T pair_proj(const T (&w)[(L + 1) * (L + 1)], const PairGeom<T, L> &G)
-{
-    return pair_dot<T, L>(w, G);
-}
-template <typename T, int L>
-__device__ __forceinline__ void pair_back(T (&acc)[(L + 1) * (L + 1)], T t, const PairRt<T, L> &G)
-{
-    pair_axpy_rt<T, L>(acc, t, G);
-}
-template <typename T, int L>
-__device__ __forceinline__ void pair_back(T (&acc)[(L + 1) * (L + 1)], T t, const PairGeom<T, L> &G)
-{
-    pair_axpy<T, L>(acc, t, G);
 }
 
 // Walks the same Siddon crossings as march_tau and calls f(v, length) for

@@ -108,11 +108,9 @@ __global__ __launch_bounds__(kWave) void k_prop(PropArgs<T> A, const SolveDev<T,
         const T kf = kfs[0];
         r2 = self ? T(1) : r2;
         const T rinv = dev_rsqrt<T>(r2);
-        // the expanded harmonics everywhere except the FP32 complex group, where
-        // their extra registers cost more than the per-side products they save
-        using Geom = typename std::conditional<CPLX && sizeof(T) == 4, PairGeom<T, L>, PairRt<T, L>>::type;
-        Geom G;
-        pair_setup<T, L>(G, ddx * rinv, ddy * rinv, ddz * rinv);
+        // the pair's expanded harmonics, formed once and shared by every right-hand side
+        PairRt<T, L> G;
+        pair_rt<T, L>(G, ddx * rinv, ddy * rinv, ddz * rinv);
         // w holds right-hand side r of source s; the next one (r+1, or 0 of
         // s+1) is requested before r is applied, hiding the scalar-load latency
         const int snext = (s + 1 < s1) ? s + 1 : s;
@@ -122,19 +120,19 @@ __global__ __launch_bounds__(kWave) void k_prop(PropArgs<T> A, const SolveDev<T,
             // One set of scalar registers holds W (see below): the imaginary
             // half is loaded after the real projection, the next source's real
             // half after the imaginary one, covered by the two axpys.
-            const T p0 = pair_proj<T, L>(w, G);
+            const T p0 = pair_dot_rt<T, L>(w, G);
 #pragma unroll
             for (int a = 0; a < K; ++a) w[a] = Wsrc[1][(size_t)s * K + a];
             T sn, cs;
             dev_sincos<T>(kim * (r2 * rinv), &sn, &cs);
             const T kc = kf * cs, ks = kf * sn;
             __builtin_amdgcn_sched_barrier(0);
-            const T p1 = pair_proj<T, L>(w, G);
+            const T p1 = pair_dot_rt<T, L>(w, G);
 #pragma unroll
             for (int a = 0; a < K; ++a) w[a] = Wsrc[0][(size_t)snext * K + a];
             __builtin_amdgcn_sched_barrier(0);
-            pair_back<T, L>(acc[0], kc * p0 + ks * p1, G);
-            pair_back<T, L>(acc[1], kc * p1 - ks * p0, G);
+            pair_axpy_rt<T, L>(acc[0], kc * p0 + ks * p1, G);
+            pair_axpy_rt<T, L>(acc[1], kc * p1 - ks * p0, G);
             __builtin_amdgcn_sched_barrier(0);
         } else if constexpr (sizeof(T) == 4 && K <= 36 && NR == 2) {
             // Groups of two: two sets of scalar registers hold W (2 x 36 floats
@@ -155,8 +153,8 @@ __global__ __launch_bounds__(kWave) void k_prop(PropArgs<T> A, const SolveDev<T,
                 for (int a = 0; a < K; ++a)
                     wn[a] = (r + 1 < NR) ? Wsrc[r + 1][(size_t)s * K + a] : Wsrc[0][(size_t)snext * K + a];
                 __builtin_amdgcn_sched_barrier(0);
-                const T t = kfs[SEP ? r : 0] * pair_proj<T, L>(w, G);
-                pair_back<T, L>(acc[r], t, G);
+                const T t = kfs[SEP ? r : 0] * pair_dot_rt<T, L>(w, G);
+                pair_axpy_rt<T, L>(acc[r], t, G);
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int a = 0; a < K; ++a) w[a] = wn[a];
@@ -169,12 +167,12 @@ __global__ __launch_bounds__(kWave) void k_prop(PropArgs<T> A, const SolveDev<T,
             // above the axpy (two live sets of W do not fit the SGPR file).
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
-                const T t = kfs[SEP ? r : 0] * pair_proj<T, L>(w, G);
+                const T t = kfs[SEP ? r : 0] * pair_dot_rt<T, L>(w, G);
 #pragma unroll
                 for (int a = 0; a < K; ++a)
                     w[a] = (r + 1 < NR) ? Wsrc[r + 1][(size_t)s * K + a] : Wsrc[0][(size_t)snext * K + a];
                 __builtin_amdgcn_sched_barrier(0);
-                pair_back<T, L>(acc[r], t, G);
+                pair_axpy_rt<T, L>(acc[r], t, G);
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
