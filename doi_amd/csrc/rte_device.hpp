@@ -25,49 +25,16 @@ __device__ __forceinline__ float dev_exp<float>(float x) { return expf(x); }
 template <>
 __device__ __forceinline__ double dev_exp<double>(double x) { return exp(x); }
 
-// FP64 sine and cosine with no branch, for |x| < kSinCosMax.  The library
-// routine switches per lane to a large-argument reduction; inside the
-// propagation loop that divergent branch is where the register allocator has
-// placed spill copies of values live in every lane under the branch's partial
-// EXEC mask, losing them in the other lanes.  Here: Cody-Waite reduction
-// x = k pi/2 + r with pi/2 = P1 + P2 + P3 (P1, P2 of 31 and 32 significant
-// bits, so k P1 and k P2 are exact for |k| < 2^21, and x - k P1 is exact),
-// then the Taylor series of sin and cos on |r| <= pi/4 to r^19 and r^20
-// (next terms below 2^-60 relative) and the quadrant by selects.  Absolute
-// error below 2^-52 on |x| < kSinCosMax; sin(-x) = -sin x, cos(-x) = cos x exactly.
-__host__ __device__ constexpr double factorial(int n) { return n <= 1 ? 1.0 : n * factorial(n - 1); }  // exact to 22!
-__host__ __device__ constexpr double inv_factorial(int n) { return 1.0 / factorial(n); }
-__host__ __device__ __forceinline__ void sincos_cw(double x, double *s, double *c)
-{
-    constexpr double P1 = 0x1.921fb544p+0, P2 = 0x1.0b4611a6p-34, P3 = 0x1.3198a2e037073p-69;
-    constexpr double TWO_OVER_PI = 0x1.45f306dc9c883p-1;
-    const double k = rint(x * TWO_OVER_PI);
-    double r = fma(-k, P1, x);
-    r = fma(-k, P2, r);
-    r = fma(-k, P3, r);
-    const double r2 = r * r;
-    // sin r = r + r^3 S(r^2), cos r = 1 + r^2 C(r^2)
-    double S = -inv_factorial(19), C = inv_factorial(20);
-#pragma unroll
-    for (int j = 8; j >= 1; --j) S = fma(S, r2, (j & 1) ? -inv_factorial(2 * j + 1) : inv_factorial(2 * j + 1));
-#pragma unroll
-    for (int j = 9; j >= 1; --j) C = fma(C, r2, (j & 1) ? -inv_factorial(2 * j) : inv_factorial(2 * j));
-    const double sr = fma(r * r2, S, r), cr = fma(r2, C, 1.0);
-    const int q = (int)k & 3;
-    const double a = (q & 1) ? cr : sr, b = (q & 1) ? sr : cr;
-    *s = (q & 2) ? -a : a;
-    *c = ((q + 1) & 2) ? -b : b;
-}
-
-// sin and cos of the pair phase k r: in FP32 the hardware sine/cosine
-// (absolute error ~1e-6 at arguments of order 10, below the FP32 path's
-// tolerance), in FP64 the branch-free routine above
+// sin and cos of the FP32 pair phase k r: the hardware sine/cosine (absolute
+// error ~1e-6 at arguments of order 10, below the FP32 path's tolerance).  The
+// FP64 kernel reads the phase from a per-displacement table instead: a library
+// sincos in its loop branches per lane to a large-argument reduction, and in that
+// heavily spilled kernel the register allocator placed copies of values live in
+// every lane into the branch's flow block, under its partial EXEC mask.
 template <typename T>
 __device__ __forceinline__ void dev_sincos(T x, T *s, T *c);
 template <>
 __device__ __forceinline__ void dev_sincos<float>(float x, float *s, float *c) { __sincosf(x, s, c); }
-template <>
-__device__ __forceinline__ void dev_sincos<double>(double x, double *s, double *c) { sincos_cw(x, s, c); }
 
 template <typename T>
 __device__ __forceinline__ T big_value();

@@ -857,6 +857,36 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
         HIP_TRY(launch_fill<TA>(dphi.as<TA>(), szW * hv, TA(0), st));
         for (int hh = 0; hh < hv; ++hh) sv[hh].phi = dphi.as<TA>() + szW * hh;
     }
+    // FP64 frequency-domain solves read the pair phase (cos, sin)(kim r) from a
+    // table over the displacements, one per distinct kim, filled with the C
+    // library's sine and cosine (the FP32 kernel evaluates it in hardware)
+    std::vector<std::unique_ptr<DevBuf>> dphase;
+    if (kim && sizeof(T) == 8) {
+        const int ex = 2 * grid.nx - 1, ey = 2 * grid.ny - 1, ez = 2 * grid.nz - 1;
+        const size_t nd = (size_t)ex * ey * ez;
+        std::vector<double> done, tab(2 * nd);
+        for (int i = 0; i < nb; ++i) {
+            size_t t = 0;
+            while (t < done.size() && done[t] != kim[i]) ++t;
+            if (t == done.size()) {
+                size_t j = 0;
+                for (int iz = 0; iz < ez; ++iz)
+                    for (int iy = 0; iy < ey; ++iy)
+                        for (int ix = 0; ix < ex; ++ix, ++j) {
+                            const double rx = (ix - (grid.nx - 1)) * dx, ry = (iy - (grid.ny - 1)) * dy,
+                                         rz = (iz - (grid.nz - 1)) * dz;
+                            const double ph = kim[i] * std::sqrt(rx * rx + ry * ry + rz * rz);
+                            tab[2 * j] = std::cos(ph);
+                            tab[2 * j + 1] = std::sin(ph);
+                        }
+                dphase.emplace_back(new DevBuf);
+                HIP_TRY(dphase.back()->alloc(sizeof(double) * tab.size()));
+                HIP_TRY(hipMemcpy(dphase.back()->p, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
+                done.push_back(kim[i]);
+            }
+            for (int hh = 0; hh < hv; ++hh) sv[i * hv + hh].phase = (const T *)dphase[t]->p;
+        }
+    }
     SolveDev<T, TA> *svd = ctx->ws_sv.as<SolveDev<T, TA>>();
     HIP_TRY(hipMemcpyAsync(svd, sv.data(), sizeof(SolveDev<T, TA>) * ns, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(ctx->ws_beams.p, bd.data(), sizeof(BeamDev) * nb, hipMemcpyHostToDevice, st));
@@ -1224,12 +1254,6 @@ extern "C" int rte_solve_fd(rte_ctx *ctx, int nb, rte_medium *const *media, cons
         if (!std::isfinite(omega[i])) return fail(RTE_E_ARG, "omega[%d] is not finite", i);
         // CPU package/src/RTE_uniform.c:126: mu_a + i modulation / (c / n)
         kim[i] = omega[i] * media[i]->n / media[i]->c;
-        // the phase e^{-i kim r} of the farthest voxel pair must stay in the sine's range
-        const double ex = (a.xmax - a.xmin) * (a.nx - 1) / a.nx, ey = (a.ymax - a.ymin) * (a.ny - 1) / a.ny,
-                     ez = (a.zmax - a.zmin) * (a.nz - 1) / a.nz;
-        if (std::fabs(kim[i]) * std::sqrt(ex * ex + ey * ey + ez * ez) >= kSinCosMax)
-            return fail(RTE_E_ARG, "omega[%d]: phase omega n / c times the grid diagonal is %g rad, above %g", i,
-                        std::fabs(kim[i]) * std::sqrt(ex * ex + ey * ey + ez * ez), kSinCosMax);
     }
     if ((rc = set_device(ctx))) return rc;
     switch (params->precision) {

@@ -15,8 +15,6 @@ constexpr int kNOmega = kAngRes * kAngRes;
 constexpr int kBrick = 4;      // destination bricks of 4x4x4 voxels = one wave64
 constexpr int kWave = 64;
 constexpr int kMaxNR = 5;      // right-hand sides (beams on one medium) per propagation wave
-// largest pair phase |omega n / c| r of a frequency-domain solve (FP64 sine/cosine range)
-constexpr double kSinCosMax = 1.0e6;
 
 // Geometry in device form.  Voxel v = x + nx*(y + ny*z); destination bricks
 // are numbered bx fastest; slot = brick*64 + lane, lane = lx + 4*(ly + 4*lz).
@@ -63,6 +61,10 @@ struct SolveDev {
     TA *phi;           // [N][K] if set, k_combine adds A W (unscaled) here: the final field
     int mode;          // 0 term, 1 init, 2 final, 3 recompute (see k_bottom / k_combine)
     T kim;             // frequency-domain solves: Im mu_t = omega n / c (uniform), 0 otherwise
+    // FP64 frequency-domain solves: (cos, sin)(kim r) per displacement
+    // (ix, iy, iz) in [1-nx, nx-1] x [1-ny, ny-1] x [1-nz, nz-1], index
+    // (ix + nx-1) + (2nx-1) ((iy + ny-1) + (2ny-1) (iz + nz-1)); else nullptr
+    const T *phase;
 };
 
 template <typename T>

@@ -84,6 +84,13 @@ __global__ __launch_bounds__(kWave) void k_prop(PropArgs<T> A, const SolveDev<T,
         for (int r = 0; r < NT; ++r)
             if (s0 < s1) kf_next[r] = ktab[r][(size_t)s0 * g.npad + slot];
     }
+    // FP64 complex group: the pair phase (cos, sin)(kim r) from the per-displacement
+    // table (read where it is used: requested a source ahead, the two values live
+    // across the loop cost this SGPR-bound kernel over 300 spill moves per pair)
+    constexpr bool PHASE_TAB = CPLX && sizeof(T) == 8;
+    const gptr<T> phase = as_global<T>(grp[0].phase);
+    const int ex = 2 * g.nx - 1, exy = ex * (2 * g.ny - 1);
+    const int dbase = (cdi + g.nx - 1) + ex * (cdj + g.ny - 1) + exy * (cdk + g.nz - 1);
     T w[K];
 #pragma unroll
     for (int a = 0; a < K; ++a) w[a] = (s0 < s1) ? Wsrc[0][(size_t)s0 * K + a] : T(0);
@@ -124,7 +131,13 @@ __global__ __launch_bounds__(kWave) void k_prop(PropArgs<T> A, const SolveDev<T,
 #pragma unroll
             for (int a = 0; a < K; ++a) w[a] = Wsrc[1][(size_t)s * K + a];
             T sn, cs;
-            dev_sincos<T>(kim * (r2 * rinv), &sn, &cs);
+            if constexpr (PHASE_TAB) {
+                const int d = dbase - si - ex * sj - exy * sk;
+                cs = phase[2 * d];
+                sn = phase[2 * d + 1];
+            } else {
+                dev_sincos<T>(kim * (r2 * rinv), &sn, &cs);
+            }
             const T kc = kf * cs, ks = kf * sn;
             __builtin_amdgcn_sched_barrier(0);
             const T p1 = pair_dot_rt<T, L>(w, G);

@@ -205,9 +205,7 @@ int rte_neumann(rte_ctx *ctx, rte_medium *m, const rte_params *params, const dou
  * stats report the real parts.  This is the series the reference's modulated
  * branch sets out to compute; that branch itself accumulates into the
  * continuous-wave field, normalises the source twice and propagates the
- * source once (RTE_uniform.c:128-160), which rte_solve_fd does not repeat.
- * RTE_E_ARG if |omega n / c| times the grid's largest voxel-centre distance
- * reaches 1e6 rad (the range of the FP64 pair phase). */
+ * source once (RTE_uniform.c:128-160), which rte_solve_fd does not repeat. */
 int rte_solve_fd(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_beam *beams, const double *omega,
                  const rte_params *params, double *exitance, double *reflectance, double *field,
                  rte_stats *stats);
