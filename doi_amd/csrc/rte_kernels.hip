@@ -173,6 +173,10 @@ __global__ __launch_bounds__(kWave) void k_prop(PropArgs<T> A, const SolveDev<T,
                 for (int a = 0; a < K; ++a) w[a] = wn[a];
             }
         } else {
+            // the harmonics complete before the first projection: interleaved with
+            // it, they pulled the wait for the row requested during the previous
+            // source's last axpy up to 69 VALU instructions after the request (120 now)
+            __builtin_amdgcn_sched_barrier(0);
             // One set of scalar registers holds W: the projection reads it, the
             // next right-hand side (or source) is then loaded into the same
             // registers, and the axpy, which needs no W, covers the load.  The
