@@ -95,3 +95,22 @@ def test_fim_scan_against_oracle(ctx, prec, tol):
             dg = snr.fim_top_image(dw, shape, bounds, L, c=c, n=nr, mag=beam[3])
             F = fim.fisher_top(g0, dg)
             assert abs(res["fisher"][i, j] - F) <= tol * abs(F), (z, r)
+
+
+def test_fim_command_line(ctx, golden_ref, tmp_path, capsys):
+    """python -m doi_amd.fim file.ini prints the reference's line with the
+    value fim_scan gives for the file's problem (z = 1, the first radius)."""
+    from doi_amd import refio
+    from tests.test_refio import ini_for
+    case = next(c for c in golden_ref if c["shape"][0] == c["shape"][1] == c["shape"][2])
+    p = tmp_path / "fim.ini"
+    p.write_text(ini_for(case))
+    assert fim.main([str(p), "--precision", "fp64"]) == 0
+    line = capsys.readouterr().out.strip().splitlines()[-1]
+    prob = refio.load(str(p))
+    params = rte.make_params(prob.L, precision=rte.PREC_FP64, stop_rule=rte.STOP_LAMBDA, max_terms=1000,
+                             prop_thresh=prob.prop_thresh)
+    res = fim.fim_scan(ctx, prob.shape, prob.bounds, prob.mu_a, prob.mu_s, prob.g, prob.beam, params, n=prob.n,
+                       c=prob.c)
+    r = fim.reference_radii(prob.shape, prob.bounds)[0]
+    assert line == "%e is fisher abs for %d as z index and %f as radius" % (res["fisher"][0, 0], 1, r)
