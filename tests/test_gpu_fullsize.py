@@ -1,7 +1,8 @@
 """GPU parity at the workloads' full sizes: the 16^3 L=5 lognormal medium against the
 committed FP64 oracle fixture, and size-independent properties at 32^3
 (mixed vs FP64 error bar, mirror symmetry, linearity in the beam power,
-batch independence)."""
+batch independence, the frequency-domain solve's symmetries) and of rte_neumann
+(the beam's own first term at 24^3, linearity in the first term at 32^3)."""
 import json
 import os
 
@@ -98,3 +99,35 @@ def test_32cube_frequency_domain_properties(ctx):
     slow, _, _ = ctx.solve_fd([m], [b], [1.0], p)       # 1 rad/s: phase ~1e-10
     assert np.max(np.abs(slow[0] - cw[0])) < 1e-6 * np.max(np.abs(cw[0]))
     m.close()
+
+
+def test_neumann_at_full_size(ctx):
+    """rte_neumann at the workloads' sizes: at 24^3, L = 5 (FP64) the beam's own
+    first term gives rte_solve_field's field and image; at 32^3, L = 5 (mixed,
+    six terms) the field is linear in the first term."""
+    n, L = 24, 5
+    ma, ms = media.lognormal_media_with_inclusions(1, seed=7, n=n)[0]
+    bounds = media.cube_bounds(n)
+    beam = media.top_face_beams(seed=7, n=n)[0]
+    print("neumann 24^3 beam source", flush=True)
+    s0, ball, _ = ctx.source((n, n, n), bounds, ma, ms, 0.9, beam, L)
+    m = ctx.medium((n, n, n), bounds, ma, ms, 0.9)
+    p = rte.make_params(L, precision=rte.PREC_FP64, stop_rule=rte.STOP_DELTA, tol=1e-6, max_terms=12)
+    field, img, _, st = ctx.solve_field(m, beam, p)
+    f, gi, _, st2 = ctx.neumann(m, s0, p)
+    m.close()
+    assert st2["terms"] == st["terms"]
+    assert rel_max(f, field) < 1e-12
+    assert rel_max(gi + np.reshape(ball, gi.shape), img) < 1e-12
+    n = 32
+    ma, ms = media.lognormal_media_with_inclusions(1, seed=8, n=n)[0]
+    m = ctx.medium((n, n, n), media.cube_bounds(n), ma, ms, 0.9)
+    rng = np.random.default_rng(3)
+    K = (L + 1) ** 2
+    s1 = rng.random((K, m.N)) * np.exp(-np.arange(K))[:, None]
+    s2 = rng.random((K, m.N)) * np.exp(-np.arange(K))[:, None]
+    p = rte.make_params(L, precision=rte.PREC_MIXED, stop_rule=rte.STOP_DELTA, tol=0.0, max_terms=6)
+    print("neumann 32^3 linearity", flush=True)
+    f1, f2, f12 = (ctx.neumann(m, s, p)[0] for s in (s1, s2, s1 + 2.0 * s2))
+    m.close()
+    assert rel_max(f12, f1 + 2.0 * f2) < 1e-5
