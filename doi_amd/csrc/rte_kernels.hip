@@ -32,14 +32,18 @@ __global__ void k_fill(T *__restrict__ p, size_t n, T v)
 
 // ---------------------------------------------------------------------------
 // Propagation: partial pair sums over one chunk of sources.
-//   grid = (nblk, nchunk, ngroups), block = 64; group z holds the NR solves
-//   sv[z*NR .. z*NR+NR-1], which share one medium (same path table and
-//   mu_t): the pair direction, its SH recurrences and kf are computed once
-//   and applied to every right-hand side.
+//   grid = (ceil(nblk / kPropWaves), nchunk, ngroups), block = 64 kPropWaves:
+//   wave i of workgroup x owns brick x kPropWaves + i.  The waves of a
+//   workgroup share one CU and walk the same sources of the same solves at
+//   the same time, so one scalar-cache fill of a W row serves all of them.
+//   Group z holds the NR solves sv[z*NR .. z*NR+NR-1], which share one medium
+//   (same path table and mu_t): the pair direction, its SH recurrences and kf
+//   are computed once and applied to every right-hand side.
 // PATH 0: kf from the per-medium table; PATH 1: marched per pair.
 // ---------------------------------------------------------------------------
+constexpr int kPropWaves = 4;
 template <typename T, typename TA, int L, int PATH, int NR, bool CPLX = false, bool SEP = false>
-__global__ __launch_bounds__(kWave) void k_prop(PropArgs<T> A, const SolveDev<T, TA> *__restrict__ sv)
+__global__ __launch_bounds__(kWave * kPropWaves) void k_prop(PropArgs<T> A, const SolveDev<T, TA> *__restrict__ sv)
 {
     static_assert(!CPLX || NR == 2, "a complex field is propagated as its two real halves");
     // SEP: the group's right-hand sides are solves on different media of one grid; they
@@ -48,7 +52,9 @@ __global__ __launch_bounds__(kWave) void k_prop(PropArgs<T> A, const SolveDev<T,
     constexpr int NT = SEP ? NR : 1;  // path tables read per pair
     constexpr int K = (L + 1) * (L + 1);
     const GridDev<T> &g = A.g;
-    const int lane = threadIdx.x, blk = blockIdx.x, chunk = blockIdx.y;
+    const int lane = threadIdx.x % kWave, blk = blockIdx.x * kPropWaves + threadIdx.x / kWave;
+    const int chunk = blockIdx.y;
+    if (blk >= g.nblk) return;  // the last workgroup's waves past the last brick (no barriers follow)
     const SolveDev<T, TA> *__restrict__ grp = sv + (size_t)blockIdx.z * NR;
     int di, dj, dk;
     lane_dest(g, blk, lane, di, dj, dk);
@@ -832,7 +838,7 @@ template <typename T, typename TA>
 hipError_t launch_prop(int L, int path, int nr, const PropArgs<T> &A, const SolveDev<T, TA> *sv, int ngroups,
                        hipStream_t st)
 {
-    dim3 grid(A.g.nblk, A.nchunk, ngroups), block(kWave);
+    dim3 grid((A.g.nblk + kPropWaves - 1) / kPropWaves, A.nchunk, ngroups), block(kWave * kPropWaves);
     // several right-hand sides per group only on the mixed-precision cached path
     constexpr bool multi = sizeof(T) == 4 && sizeof(TA) == 8;
     if (nr < 1 || nr > kMaxNR || (nr > 1 && (!multi || path != 0))) return hipErrorInvalidValue;
@@ -858,7 +864,7 @@ template <typename T, typename TA>
 hipError_t launch_prop_sep(int L, int nr, const PropArgs<T> &A, const SolveDev<T, TA> *sv, int ngroups,
                            hipStream_t st)
 {
-    dim3 grid(A.g.nblk, A.nchunk, ngroups), block(kWave);
+    dim3 grid((A.g.nblk + kPropWaves - 1) / kPropWaves, A.nchunk, ngroups), block(kWave * kPropWaves);
     if constexpr (sizeof(T) == 4 && sizeof(TA) == 8) {
         if (nr != 2 && nr != 3) return hipErrorInvalidValue;
         RTE_DISPATCH_L(L, {
@@ -877,7 +883,7 @@ template <typename T, typename TA>
 hipError_t launch_prop_c(int L, int path, const PropArgs<T> &A, const SolveDev<T, TA> *sv, int ncomplex,
                          hipStream_t st)
 {
-    dim3 grid(A.g.nblk, A.nchunk, ncomplex), block(kWave);
+    dim3 grid((A.g.nblk + kPropWaves - 1) / kPropWaves, A.nchunk, ncomplex), block(kWave * kPropWaves);
     RTE_DISPATCH_L(L, {
         if (path == 1)
             hipLaunchKernelGGL((k_prop<T, TA, LL, 1, 2, true>), grid, block, 0, st, A, sv);
