@@ -376,16 +376,19 @@ __global__ void k_src_grad(int nx, int ny, int nz, double dxd, double dyd, doubl
 // ---------------------------------------------------------------------------
 // The reference's forward-mode derivative (FisherInfo_IO/src/compute_grad_rte.cu:
 // 273-301 mu_a, 341-371 mu_s): the first term of solve i is its parameter's
-// region of the forward field phi = sum_q A W_q times -1 (mu_a) or g^l - 1
-// (mu_s), in scaled coefficients; its pixel buffers start at zero (no
-// ballistic term) with the bottom-layer stop-rule sums of that first term.
+// region of the forward field phi = sum_q A W_q times -fs (mu_a) or
+// fs (g^l - 1) (mu_s), in scaled coefficients; its pixel buffers start at zero
+// (no ballistic term) with the bottom-layer stop-rule sums of that first term.
+// fs = n: the reference scales its field out2 = phi / c_m by -1.0*C and
+// (g^l - 1) C with C the vacuum speed (FisherInfo_IO/src/rte.h:15,
+// compute_grad_rte.cu:292,350-353), which is phi times C / c_m = n here.
 // wrt 2 with part == nullptr: phi itself, every voxel (a given first term,
 // the reference's Neumann(src, out, 1), FisherInfo_IO/src/neumann.cu:23).
 // grid = (ceil(N/64), nsolve), block = 64.
 // ---------------------------------------------------------------------------
 template <typename T, typename TA, int L>
 __global__ __launch_bounds__(kWave) void k_ref_source(GridDev<T> g, Scales sc, const int *__restrict__ part,
-                                                     const int *__restrict__ param, int wrt, TA gan,
+                                                     const int *__restrict__ param, int wrt, TA gan, TA fs,
                                                      const TA *__restrict__ phi, const TA *__restrict__ wsum,
                                                      const SolveDev<T, TA> *__restrict__ sv)
 {
@@ -398,7 +401,7 @@ __global__ __launch_bounds__(kWave) void k_ref_source(GridDev<T> g, Scales sc, c
     TA gl = TA(1), bs = TA(0);
 #pragma unroll
     for (int l = 0; l <= L; ++l) {
-        const TA f = wrt == 0 ? TA(-1) : wrt == 1 ? gl - TA(1) : TA(1);
+        const TA f = wrt == 0 ? -fs : wrt == 1 ? fs * (gl - TA(1)) : TA(1);
 #pragma unroll
         for (int m = -l; m <= l; ++m) {
             const int a = l * l + l + m;
@@ -522,12 +525,12 @@ hipError_t launch_src_grad(int L, int nx, int ny, int nz, double dx, double dy, 
 
 template <typename T, typename TA>
 hipError_t launch_ref_source(int L, const GridDev<T> &g, const Scales &sc, const int *part, const int *param, int wrt,
-                             TA gan, const TA *phi, const TA *wsum, const SolveDev<T, TA> *sv, int nsolve,
+                             TA gan, TA fs, const TA *phi, const TA *wsum, const SolveDev<T, TA> *sv, int nsolve,
                              hipStream_t st)
 {
     dim3 grid((g.N + kWave - 1) / kWave, nsolve), block(kWave);
     RTE_DISPATCH_L2(L, hipLaunchKernelGGL((k_ref_source<T, TA, LL>), grid, block, 0, st, g, sc, part, param, wrt,
-                                          gan, phi, wsum, sv));
+                                          gan, fs, phi, wsum, sv));
     return hipGetLastError();
 }
 
@@ -539,10 +542,10 @@ hipError_t launch_accumulate(size_t n, const TA *x, TA *y, hipStream_t st)
 }
 
 template hipError_t launch_ref_source<float, double>(int, const GridDev<float> &, const Scales &, const int *,
-                                                     const int *, int, double, const double *, const double *,
+                                                     const int *, int, double, double, const double *, const double *,
                                                      const SolveDev<float, double> *, int, hipStream_t);
 template hipError_t launch_ref_source<double, double>(int, const GridDev<double> &, const Scales &, const int *,
-                                                      const int *, int, double, const double *, const double *,
+                                                      const int *, int, double, double, const double *, const double *,
                                                       const SolveDev<double, double> *, int, hipStream_t);
 template hipError_t launch_accumulate<double>(size_t, const double *, double *, hipStream_t);
 template hipError_t launch_adj_prop<float, double>(int, const PropArgs<float> &, const float *, const float *,

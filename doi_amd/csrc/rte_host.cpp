@@ -21,6 +21,7 @@
 #include <set>
 #include <string>
 #include <tuple>
+#include <unordered_set>
 #include <vector>
 
 #include "../../include/rte.h"
@@ -758,6 +759,7 @@ struct RefSrc {
     int wrt = 0;                 // 0 mu_a, 1 mu_s
     const TA *phi = nullptr;     // device, [N][K]: sum_q A W_q of the forward solve
     const TA *wsum = nullptr;    // device, K: stop-rule weights
+    double *cplx_field = nullptr; // host, [N][K] complex: the final field in the reference's layout
 };
 
 template <typename T, typename TA>
@@ -852,7 +854,8 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
     }
     // the final field sum_q A W_q of solve 0 (both halves when complex), if asked for
     DevBuf dphi;
-    if (field) {
+    const bool want_cplx = rs && rs->cplx_field;
+    if (field || want_cplx) {
         HIP_TRY(dphi.alloc(szW * hv * sizeof(TA)));
         HIP_TRY(launch_fill<TA>(dphi.as<TA>(), szW * hv, TA(0), st));
         for (int hh = 0; hh < hv; ++hh) sv[hh].phi = dphi.as<TA>() + szW * hh;
@@ -895,8 +898,8 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
         LaunchTimer tm(ctx, 2);
         if (rs) {
             if constexpr (sizeof(TA) == 8) {
-                HIP_TRY(launch_ref_source<T, TA>(L, g, sc, rs->part, rs->param, rs->wrt, TA(T(media[0]->g)), rs->phi,
-                                                 rs->wsum, svd, nb, st));
+                HIP_TRY(launch_ref_source<T, TA>(L, g, sc, rs->part, rs->param, rs->wrt, TA(T(media[0]->g)),
+                                                 TA(media[0]->n), rs->phi, rs->wsum, svd, nb, st));
             } else {
                 return fail(RTE_E_UNSUPP, "derivative solves need FP64 running sums");
             }
@@ -1070,6 +1073,14 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
             for (int q = 0; q < P; ++q) reflectance[(size_t)q0 * P + q] = (double)img[q];
         }
     }
+    if (want_cplx) {
+        // the reference's complex basis and layout, converted on the device
+        DevBuf dC;
+        HIP_TRY(dC.alloc(sizeof(double) * 2 * szW));
+        HIP_TRY(launch_real_to_cplx<TA>(L, N, media[0]->n / media[0]->c, dphi.as<TA>(), dC.as<double>(), st));
+        HIP_TRY(hipMemcpyAsync(rs->cplx_field, dC.p, sizeof(double) * 2 * szW, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
     if (field) {
         // coefficient-major, in the reference's units (A carries 1/c_m there)
         std::vector<TA> f(szW * hv);
@@ -1231,6 +1242,99 @@ extern "C" int rte_neumann(rte_ctx *ctx, rte_medium *m, const rte_params *params
     if (params->precision == RTE_PREC_FP64)
         return neumann_run<double, double>(ctx, m, params, src, field, exitance, reflectance, stats);
     return neumann_run<float, double>(ctx, m, params, src, field, exitance, reflectance, stats);
+}
+
+template <typename T, typename TA>
+static int propagate_impl(rte_ctx *ctx, rte_medium *m, int L, double prop_thresh, int path,
+                          const double *W, double *out, bool cplx = false);
+
+// Neumann(src, out, flag) on the reference's own field layout: the complex field
+// is converted to the real basis on the device, run through the series of
+// neumann_run (flag 1) or one propagation (flag 0), and converted back there.
+template <typename T, typename TA>
+static int neumann_ref_run(rte_ctx *ctx, rte_medium *m, const rte_params *p, const double *src, double *out,
+                           rte_stats *stats)
+{
+    const int L = p->L, K = (L + 1) * (L + 1), N = m->N, P = m->grid.nx * m->grid.ny;
+    hipStream_t st = ctx->stream;
+    std::vector<TA> ws(K);
+    for (int a = 0; a < K; ++a) ws[a] = TA(stop_weight(a));
+    const int param0 = 0;
+    DevBuf dC, dSrc, dParam, dWs;
+    HIP_TRY(dC.alloc(sizeof(double) * 2 * K * (size_t)N));
+    HIP_TRY(dSrc.alloc(sizeof(TA) * K * (size_t)N));
+    HIP_TRY(dParam.alloc(sizeof(int)));
+    HIP_TRY(dWs.alloc(sizeof(TA) * K));
+    HIP_TRY(hipMemcpyAsync(dC.p, src, sizeof(double) * 2 * K * (size_t)N, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(dParam.p, &param0, sizeof(int), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(dWs.p, ws.data(), sizeof(TA) * K, hipMemcpyHostToDevice, st));
+    // the first term is the same in the reference's units and this library's
+    HIP_TRY(launch_cplx_to_real<TA>(L, N, 1.0, unit_scales(), dC.as<double>(), dSrc.as<TA>(), st));
+    RefSrc<TA> rs;
+    rs.part = nullptr;
+    rs.param = dParam.as<int>();
+    rs.wrt = 2;
+    rs.phi = dSrc.as<TA>();
+    rs.wsum = dWs.as<TA>();
+    rs.cplx_field = out;
+    std::vector<double> img(P);
+    rte_medium *const media[1] = {m};
+    int rc = run_batch<T, TA>(ctx, 1, media, nullptr, p, img.data(), stats, &rs);
+    HIP_TRY(hipStreamSynchronize(st));  // the uploads' host buffers end here
+    return rc;
+}
+
+extern "C" int rte_neumann_ref(rte_ctx *ctx, rte_medium *m, const rte_params *params, const double *src,
+                               double *out, int flag, rte_stats *stats)
+{
+    if (!ctx || !m || !out) return fail(RTE_E_ARG, "rte_neumann_ref: NULL argument");
+    if (m->ctx != ctx) return fail(RTE_E_ARG, "rte_neumann_ref: medium belongs to another context");
+    if (flag != 0 && flag != 1) return fail(RTE_E_ARG, "rte_neumann_ref: flag must be 0 or 1");
+    if (flag == 1 && !src) return fail(RTE_E_ARG, "rte_neumann_ref: flag 1 needs src");
+    int rc;
+    if ((rc = check_params(params))) return rc;
+    if (params->precision != RTE_PREC_FP64 && params->precision != RTE_PREC_MIXED)
+        return fail(RTE_E_UNSUPP, "rte_neumann_ref: precision must be RTE_PREC_FP64 or RTE_PREC_MIXED");
+    const int L = params->L, K = (L + 1) * (L + 1), N = m->N;
+    const double *f = flag == 1 ? src : out;
+    // a real radiance: Im c_l0 = 0, c_l,-m = (-1)^m conj c_lm (to rounding)
+    double big = 0.0, bad = 0.0;
+    for (size_t i = 0; i < (size_t)2 * K * N; ++i) {
+        if (!std::isfinite(f[i])) return fail(RTE_E_ARG, "rte_neumann_ref: field value %zu is not finite", i);
+        big = std::max(big, std::fabs(f[i]));
+    }
+    for (int v = 0; v < N; ++v)
+        for (int l = 0; l <= L; ++l) {
+            const double *c = f + (size_t)v * K * 2;
+            bad = std::max(bad, std::fabs(c[2 * (l * l + l) + 1]));
+            for (int mm = 1; mm <= l; ++mm) {
+                const double sg = (mm & 1) ? -1.0 : 1.0;
+                const int ap = l * l + l + mm, an = l * l + l - mm;
+                bad = std::max(bad, std::fabs(c[2 * an] - sg * c[2 * ap]));
+                bad = std::max(bad, std::fabs(c[2 * an + 1] + sg * c[2 * ap + 1]));
+            }
+        }
+    if (bad > 1e-12 * big)
+        return fail(RTE_E_ARG, "rte_neumann_ref: the field is not that of a real radiance "
+                               "(c_l,-m != (-1)^m conj c_lm by %g of max %g)", bad, big);
+    if (flag == 1 && out != src)
+        for (size_t i = 0; i < (size_t)2 * K * N; ++i)
+            if (out[i] != src[i])
+                return fail(RTE_E_ARG, "rte_neumann_ref: flag 1 needs out preset to src, as the reference's "
+                                       "callers do");
+    if ((rc = set_device(ctx))) return rc;
+    if (flag == 0) {
+        std::vector<double> in(f, f + (size_t)2 * K * N);
+        const double thr = params->prop_thresh;
+        if (params->precision == RTE_PREC_FP64)
+            rc = propagate_impl<double, double>(ctx, m, L, thr, params->path_mode, in.data(), out, true);
+        else
+            rc = propagate_impl<float, double>(ctx, m, L, thr, params->path_mode, in.data(), out, true);
+        if (rc == RTE_OK && stats) std::memset(stats, 0, sizeof *stats);  // no series ran
+        return rc;
+    }
+    if (params->precision == RTE_PREC_FP64) return neumann_ref_run<double, double>(ctx, m, params, src, out, stats);
+    return neumann_ref_run<float, double>(ctx, m, params, src, out, stats);
 }
 
 extern "C" int rte_solve_fd(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_beam *beams,
@@ -1653,7 +1757,8 @@ static int gram_mfma(rte_ctx *ctx, int P, int n, const float *J, const float *w,
 // parameters (FisherInfo_IO/src/compute_grad_rte.cu:273-301 mu_a, 341-371 mu_s;
 // GPU package/src/inverse_rte.cu:290-433): one forward solve keeping
 // phi = sum_q A W_q, then one Neumann series per parameter from phi restricted
-// to the parameter's voxels (times -1 for mu_a, g^l - 1 for mu_s), all
+// to the parameter's voxels (times -n for mu_a, n (g^l - 1) for mu_s: the
+// reference's -1.0*C and (g^l - 1) C on its field phi / c_m), all
 // parameters as one batch on the shared medium; column a of J is that series'
 // exitance without the ballistic term.  Not the exact derivative of the
 // discrete model: the reference approximates dA/dmu by -A chi A.
@@ -1822,8 +1927,9 @@ static int gram_mfma(rte_ctx *ctx, int P, int n, const float *J, const float *w,
 // ---------------------------------------------------------------------------
 template <typename T, typename TA>
 static int propagate_impl(rte_ctx *ctx, rte_medium *m, int L, double prop_thresh, int path,
-                          const double *W, double *out)
+                          const double *W, double *out, bool cplx)
 {
+    // cplx: W and out in the reference's layout (complex basis, voxel-major, 2 K N doubles)
     const rte_grid &grid = m->grid;
     const int K = (L + 1) * (L + 1);
     const GridDev<T> g = make_grid<T>(grid);
@@ -1840,15 +1946,19 @@ static int propagate_impl(rte_ctx *ctx, rte_medium *m, int L, double prop_thresh
     const int clen = (N + nch0 - 1) / nch0;
     const int nch = (N + clen - 1) / clen;
     DevBuf dIn, dW, dPart, dAW, dOut, dSv;
-    HIP_TRY(dIn.alloc((size_t)K * N * 8));
+    const size_t nio = (size_t)K * N * (cplx ? 2 : 1);
+    HIP_TRY(dIn.alloc(nio * 8));
     HIP_TRY(dW.alloc((size_t)K * N * sizeof(T)));
     HIP_TRY(dPart.alloc((size_t)nch * N * K * sizeof(T)));
     HIP_TRY(dAW.alloc((size_t)K * N * sizeof(TA)));
-    HIP_TRY(dOut.alloc((size_t)K * N * 8));
+    HIP_TRY(dOut.alloc(nio * 8));
     HIP_TRY(dSv.alloc(sizeof(SolveDev<T, TA>)));
-    HIP_TRY(hipMemcpyAsync(dIn.p, W, (size_t)K * N * 8, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(dIn.p, W, nio * 8, hipMemcpyHostToDevice, st));
     const Scales sc = make_scales(L);
-    HIP_TRY(launch_to_voxel_major<T>(K, N, sc, dIn.as<double>(), dW.as<T>(), st));
+    if (cplx)
+        HIP_TRY(launch_cplx_to_real<T>(L, N, 1.0, sc, dIn.as<double>(), dW.as<T>(), st));
+    else
+        HIP_TRY(launch_to_voxel_major<T>(K, N, sc, dIn.as<double>(), dW.as<T>(), st));
     SolveDev<T, TA> s;
     std::memset(&s, 0, sizeof s);
     s.W = dW.as<T>();
@@ -1876,11 +1986,17 @@ static int propagate_impl(rte_ctx *ctx, rte_medium *m, int L, double prop_thresh
     ca.aw_out = dAW.as<TA>();
     for (int a = 0; a < kMaxK; ++a) ca.nsc[a] = TA(sc.n[a]);
     HIP_TRY(launch_combine<T, TA>(L, ca, dSv.as<SolveDev<T, TA>>(), 1, st));
+    // the reference's A carries 1/c_m (CPU package/src/Kernels_selfsub.c:948,1046)
+    const double cm = m->c / m->n;
+    if (cplx) {
+        HIP_TRY(launch_real_to_cplx<TA>(L, N, 1.0 / cm, dAW.as<TA>(), dOut.as<double>(), st));
+        HIP_TRY(hipMemcpyAsync(out, dOut.p, nio * 8, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        return RTE_OK;
+    }
     HIP_TRY(launch_to_coef_major<TA>(K, N, unit_scales(), dAW.as<TA>(), dOut.as<double>(), st));
     HIP_TRY(hipMemcpyAsync(out, dOut.p, (size_t)K * N * 8, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
-    // the reference's A carries 1/c_m (CPU package/src/Kernels_selfsub.c:948,1046)
-    const double cm = m->c / m->n;
     for (size_t i = 0; i < (size_t)K * N; ++i) out[i] /= cm;
     return RTE_OK;
 }
@@ -2628,10 +2744,13 @@ static void server_loop(rte_server *S)
             admitted.clear();
             if (rc) {
                 // a device error ends the service: every request not yet finished fails
+                // (those this pass already finished keep their outcome and are counted once)
                 S->worker_rc = rc;
                 S->failed = true;
+                const std::unordered_set<ServReq *> done_now(finished.begin(), finished.end());
                 for (auto &kv : S->reqs)
-                    if (kv.second->state == REQ_RUNNING || kv.second->state == REQ_QUEUED) {
+                    if ((kv.second->state == REQ_RUNNING || kv.second->state == REQ_QUEUED) &&
+                        !done_now.count(kv.second.get())) {
                         kv.second->rc = rc;
                         kv.second->err = g_err;
                         kv.second->outcome = REQ_FAILED;

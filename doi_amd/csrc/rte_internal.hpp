@@ -147,6 +147,13 @@ hipError_t launch_to_voxel_major(int K, int N, const Scales &sc, const double *i
 // in: voxel-major TA [N][K] -> out: coefficient-major double [K][N], divided by sc.n[a]
 template <typename TA>
 hipError_t launch_to_coef_major(int K, int N, const Scales &sc, const TA *in, double *out, hipStream_t st);
+// the reference's field layout (complex SH basis, voxel-major [N][K], re/im interleaved,
+// a real radiance) <-> the real basis: out = voxel-major T [N][K], times s sc.n[a]
+template <typename T>
+hipError_t launch_cplx_to_real(int L, int N, double s, const Scales &sc, const double *c, T *out, hipStream_t st);
+// voxel-major TA [N][K] real basis, times s -> the reference's layout
+template <typename TA>
+hipError_t launch_real_to_cplx(int L, int N, double s, const TA *w, double *c, hipStream_t st);
 hipError_t launch_scatter_op(int L, int N, double g, const double *mus, double *W, hipStream_t st);
 hipError_t launch_exitance_op(int L, int nx, int ny, int nz, const double *coefE, const double *w,
                               double *img, hipStream_t st);
@@ -187,7 +194,7 @@ hipError_t launch_adj_init(int L, const GridDev<T> &g, const TA *cot, const Scal
                            hipStream_t st);
 template <typename T, typename TA>
 hipError_t launch_ref_source(int L, const GridDev<T> &g, const Scales &sc, const int *part, const int *param, int wrt,
-                             TA gan, const TA *phi, const TA *wsum, const SolveDev<T, TA> *sv, int nsolve,
+                             TA gan, TA fs, const TA *phi, const TA *wsum, const SolveDev<T, TA> *sv, int nsolve,
                              hipStream_t st);
 template <typename TA>
 hipError_t launch_accumulate(size_t n, const TA *x, TA *y, hipStream_t st);

@@ -727,6 +727,61 @@ __global__ void k_to_coef_major(int K, int N, Scales sc, const TA *__restrict__ 
     out[i] = (double)in[(size_t)v * K + a] / sc.n[a];
 }
 
+// The reference's fields are complex coefficients of the complex orthonormal
+// harmonics (Condon-Shortley phase, Y_l,-m = (-1)^m conj Y_lm), voxel-major
+// W[v K + cnt], cnt = l^2 + l + m (GPU package/src/defines.h:59).  For a real
+// radiance c_l,-m = (-1)^m conj c_lm, and in the real basis R_lm = sqrt2 Re Y_lm,
+// R_l,-m = sqrt2 Im Y_lm (m > 0) its coefficients are
+//   r_l0 = c_l0,  r_lm = sqrt2 Re c_lm,  r_l,-m = -sqrt2 Im c_lm,
+// and back  c_lm = (r_lm - i r_l,-m)/sqrt2,  c_l,-m = (-1)^m (r_lm + i r_l,-m)/sqrt2.
+// One thread per (voxel, coefficient) of the output.
+template <typename T>
+__global__ void k_cplx_to_real(int L, int N, double s, Scales sc, const double *__restrict__ c,
+                               T *__restrict__ out)
+{
+    const int K = (L + 1) * (L + 1);
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)K * N) return;
+    const int v = (int)(i / K), a = (int)(i % K);
+    int l = 0;
+    while ((l + 1) * (l + 1) <= a) ++l;
+    const int m = a - l * l - l;
+    const double *cv = c + (size_t)v * K * 2;
+    double r;
+    if (m == 0)
+        r = cv[2 * a];
+    else if (m > 0)
+        r = M_SQRT2 * cv[2 * a];
+    else
+        r = -M_SQRT2 * cv[2 * (l * l + l - m) + 1];  // Im c_l,|m|
+    out[i] = T(r * s * sc.n[a]);
+}
+
+template <typename TA>
+__global__ void k_real_to_cplx(int L, int N, double s, const TA *__restrict__ w, double *__restrict__ c)
+{
+    const int K = (L + 1) * (L + 1);
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)K * N) return;
+    const int v = (int)(i / K), a = (int)(i % K);
+    int l = 0;
+    while ((l + 1) * (l + 1) <= a) ++l;
+    const int m = a - l * l - l, am = m < 0 ? -m : m;
+    const TA *wv = w + (size_t)v * K;
+    double re, im;
+    if (m == 0) {
+        re = (double)wv[a];
+        im = 0.0;
+    } else {
+        const double rc = (double)wv[l * l + l + am], rs = (double)wv[l * l + l - am];
+        const double sg = (m < 0 && (am & 1)) ? -1.0 : 1.0;  // (-1)^m for the negative order
+        re = sg * rc * M_SQRT1_2;
+        im = (m > 0 ? -rs : sg * rs) * M_SQRT1_2;
+    }
+    c[2 * i] = re * s;
+    c[2 * i + 1] = im * s;
+}
+
 // D on a coefficient-major double field: W_a(v) *= g^l mu_s(v)
 __global__ void k_scatter_op(int L, int N, double g, const double *__restrict__ mus,
                              double *__restrict__ W)
@@ -995,6 +1050,22 @@ hipError_t launch_to_coef_major(int K, int N, const Scales &sc, const TA *in, do
     return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_cplx_to_real(int L, int N, double s, const Scales &sc, const double *c, T *out, hipStream_t st)
+{
+    const size_t n = (size_t)(L + 1) * (L + 1) * N;
+    hipLaunchKernelGGL(k_cplx_to_real<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, L, N, s, sc, c, out);
+    return hipGetLastError();
+}
+
+template <typename TA>
+hipError_t launch_real_to_cplx(int L, int N, double s, const TA *w, double *c, hipStream_t st)
+{
+    const size_t n = (size_t)(L + 1) * (L + 1) * N;
+    hipLaunchKernelGGL(k_real_to_cplx<TA>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, L, N, s, w, c);
+    return hipGetLastError();
+}
+
 hipError_t launch_scatter_op(int L, int N, double g, const double *mus, double *W, hipStream_t st)
 {
     size_t n = (size_t)(L + 1) * (L + 1) * N;
@@ -1081,6 +1152,12 @@ template hipError_t launch_source<float, float>(int, const GridDev<float> &, dou
 template hipError_t launch_to_voxel_major<float>(int, int, const Scales &, const double *, float *, hipStream_t);
 template hipError_t launch_to_voxel_major<double>(int, int, const Scales &, const double *, double *,
                                                   hipStream_t);
+template hipError_t launch_cplx_to_real<float>(int, int, double, const Scales &, const double *, float *,
+                                               hipStream_t);
+template hipError_t launch_cplx_to_real<double>(int, int, double, const Scales &, const double *, double *,
+                                                hipStream_t);
+template hipError_t launch_real_to_cplx<float>(int, int, double, const float *, double *, hipStream_t);
+template hipError_t launch_real_to_cplx<double>(int, int, double, const double *, double *, hipStream_t);
 template hipError_t launch_to_coef_major<float>(int, int, const Scales &, const float *, double *, hipStream_t);
 template hipError_t launch_to_coef_major<double>(int, int, const Scales &, const double *, double *,
                                                  hipStream_t);

@@ -9,8 +9,8 @@ image of the reference's forward-mode derivative with respect to mu_a of the
 voxels whose centres lie within `radius` of voxel (nx/2, ny/2, z), inclusive
 (rte_fim.cu:256-273 zeroes the field where the squared distance exceeds
 radius^2, :265): one Neumann series (rte_neumann) from the forward field restricted
-to that region, times -c_m (rte_fim.cu:274-277; -C in the reference's units,
-whose A carries no 1/c_m).  The reference's radii are (i + 1) fov / (2 nx),
+to that region, times -C (rte_fim.cu:274-277), C the vacuum speed of light
+(FisherInfo_IO/src/rte.h:15; the optics' c here).  The reference's radii are (i + 1) fov / (2 nx),
 i = 0..4, fov = xmax - xmin (rte_fim.cu:209-216), its depths z = 1 .. nz - 3
 (the loops at :252-253, commented out: the program runs z = initz = 1 and the
 smallest radius, :254-255, which holds the centre voxel alone).  It sets every
@@ -69,7 +69,7 @@ def fim_scan(ctx, shape, bounds, mua, mus, g, beam, params, n=1.0, c=2.9979e10, 
         for a, z in enumerate(depths):
             for b, rad in enumerate(radii):
                 inside = region_mask(shape, bounds, (nx // 2, ny // 2, z), rad)
-                src = np.where(inside[None, :], -(c / n) * field, 0.0)
+                src = np.where(inside[None, :], -c * field, 0.0)
                 dfield, _, _, _ = ctx.neumann(m, src, params)
                 dg = fim_top_image(dfield, shape, bounds, params.L, c=c, n=n, mag=beam[3])
                 dgs[(z, rad)] = dg

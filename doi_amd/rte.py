@@ -106,6 +106,7 @@ SIGNATURES = {
                                    _dp, _dp, C.POINTER(Stats)]),
     "rte_solve_field": (C.c_int, [_vp, _vp, C.POINTER(Beam), C.POINTER(Params), _dp, _dp, _dp, C.POINTER(Stats)]),
     "rte_neumann": (C.c_int, [_vp, _vp, C.POINTER(Params), _dp, _dp, _dp, _dp, C.POINTER(Stats)]),
+    "rte_neumann_ref": (C.c_int, [_vp, _vp, C.POINTER(Params), _dp, _dp, C.c_int, C.POINTER(Stats)]),
     "rte_solve_fd": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), C.POINTER(Beam), _dp, C.POINTER(Params), _dp, _dp, _dp,
                                C.POINTER(Stats)]),
     "rte_exitance_vjp": (C.c_int, [_vp, _vp, C.POINTER(Beam), C.POINTER(Params), C.POINTER(GradOpts), _dp, _dp,
@@ -163,6 +164,37 @@ def _p(a):
 
 def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def to_reference_layout(field, L):
+    """Real-basis coefficient-major field [K, N] -> the reference's layout: complex
+    coefficients of the complex harmonics (Y_l,-m = (-1)^m conj Y_lm), voxel-major
+    [N, K] (GPU package/src/defines.h:59).  c_l0 = r_l0, c_lm = (r_lm - i r_l,-m)/sqrt2,
+    c_l,-m = (-1)^m conj c_lm (m > 0); rte_neumann_ref does the same on the device."""
+    K = (L + 1) ** 2
+    r = np.asarray(field, dtype=np.float64).reshape(K, -1)
+    c = np.zeros(r.shape, dtype=np.complex128)
+    for l in range(L + 1):
+        c[l * l + l] = r[l * l + l]
+        for m in range(1, l + 1):
+            cp = (r[l * l + l + m] - 1j * r[l * l + l - m]) / np.sqrt(2.0)
+            c[l * l + l + m] = cp
+            c[l * l + l - m] = (-1) ** m * np.conj(cp)
+    return np.ascontiguousarray(c.T)
+
+
+def from_reference_layout(cfield, L):
+    """The inverse of to_reference_layout for the field of a real radiance:
+    [N, K] complex -> [K, N] real (r_lm = sqrt2 Re c_lm, r_l,-m = -sqrt2 Im c_lm)."""
+    K = (L + 1) ** 2
+    c = np.asarray(cfield, dtype=np.complex128).reshape(-1, K).T
+    r = np.zeros(c.shape)
+    for l in range(L + 1):
+        r[l * l + l] = c[l * l + l].real
+        for m in range(1, l + 1):
+            r[l * l + l + m] = np.sqrt(2.0) * c[l * l + l + m].real
+            r[l * l + l - m] = -np.sqrt(2.0) * c[l * l + l + m].imag
+    return r
 
 
 def make_params(L, precision=PREC_MIXED, stop_rule=STOP_LAMBDA, tol=1e-6, max_terms=1000,
@@ -289,6 +321,23 @@ class Context:
         _check(lib().rte_neumann(self.h, medium.h, C.byref(params), _p(src), _p(field), _p(img), _p(refl),
                                  C.byref(st)))
         return field.reshape(K, medium.N), img.reshape(ny, nx), refl.reshape(ny, nx), _stats_dict(st)
+
+    def neumann_ref(self, medium, src, params, flag=1, out=None):
+        """The reference's Neumann(src, out, flag) on its own layout (rte_neumann_ref):
+        src and out complex [N, K] (voxel-major, complex SH basis, reference units).
+        flag 1: out defaults to a copy of src, as the reference's callers preset it,
+        and receives A sum_q (DA)^q src; flag 0: out <- A out.  Returns (out, stats)."""
+        K = (params.L + 1) ** 2
+        if flag == 1:
+            src = np.ascontiguousarray(np.reshape(src, (medium.N, K)), dtype=np.complex128)
+            out = src.copy() if out is None else np.ascontiguousarray(out, dtype=np.complex128).copy()
+        else:
+            out = np.ascontiguousarray(np.reshape(out, (medium.N, K)), dtype=np.complex128).copy()
+        st = Stats()
+        sp = src.ctypes.data_as(_dp) if flag == 1 else None
+        _check(lib().rte_neumann_ref(self.h, medium.h, C.byref(params), sp, out.ctypes.data_as(_dp), flag,
+                                     C.byref(st)))
+        return out.reshape(medium.N, K), _stats_dict(st)
 
     def solve_fd(self, media, beams, omegas, params, field=False):
         """Frequency-domain solves (rte_solve_fd): complex transmitted and

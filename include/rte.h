@@ -190,6 +190,25 @@ int rte_solve_field(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte
 int rte_neumann(rte_ctx *ctx, rte_medium *m, const rte_params *params, const double *src, double *field,
                 double *exitance, double *reflectance, rte_stats *stats);
 
+/* The reference's Neumann(src, out, flag) on its own field layout
+ * (FisherInfo_IO/src/neumann.cu:23-120, FisherInfo_IO/src/rte.h:187; GPU package/src/
+ * neumann.cu:23-181): src and out are K*N complex_double (2*K*N doubles, re then im) of
+ * the reference's complex SH basis (CPU package/src/SphHarm.c:113-160: orthonormal,
+ * Condon-Shortley phase, Y_l,-m = (-1)^m conj Y_lm), voxel-major W[v*K + cnt],
+ * cnt = l*l + l + m (GPU package/src/defines.h:59), in the reference's units.  The
+ * field must be that of a real radiance (c_l,-m = (-1)^m conj c_lm, Im c_l0 = 0, to
+ * 1e-12 of its largest value), as every field of the reference's continuous-wave
+ * solves is; others are rejected (RTE_E_ARG).  Both layouts are converted on the device.
+ * flag 1: the whole series, out <- A sum_q (D A)^q src until the stop rule, with out
+ *   preset to src on entry as every reference call site does (GPU package/src/
+ *   rte.cu:140-174, FisherInfo_IO/src/rte_fim.cu:235-237, 275-276,
+ *   compute_grad_rte.cu:244-249, 297-299; anything else is RTE_E_ARG); src is left
+ *   unchanged (the reference leaves the last term in it);
+ * flag 0: the final absorption alone, out <- A out (src unused, may be NULL).
+ * RTE_PREC_FP64 or RTE_PREC_MIXED. */
+int rte_neumann_ref(rte_ctx *ctx, rte_medium *m, const rte_params *params, const double *src, double *out,
+                    int flag, rte_stats *stats);
+
 /* Frequency-domain solves of intensity-modulated beams: omega[i] (rad/s) is
  * the reference's [Source] modulation, which turns mu_a into
  * mu_a + i omega / (c/n) (CPU package/src/RTE_uniform.c:120-177, Inits.c:60).
@@ -250,9 +269,12 @@ int rte_fisher(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte_para
 
 /* The reference's own derivative of the exitance with respect to block parameters
  * (FisherInfo_IO/src/compute_grad_rte.cu:273-301 mu_a, 341-371 mu_s; GPU package/src/
- * inverse_rte.cu:290-433): phi = A sum_q W_q of the forward solve; per parameter a, one
- * Neumann series from c_m (-1) phi (wrt 0) or c_m (g^l - 1) phi (wrt 1) on the voxels with
- * partition[v] == a, and J[p][a] = its exitance without the ballistic term (nx*ny x nparam).
+ * inverse_rte.cu:290-433): phi = A sum_q W_q of the forward solve in the reference's units;
+ * per parameter a, one Neumann series from -C phi (wrt 0) or C (g^l - 1) phi (wrt 1) on the
+ * voxels with partition[v] == a, C = optics.c, the reference's literal -1.0*C with its C the
+ * vacuum speed (FisherInfo_IO/src/rte.h:15, compute_grad_rte.cu:292,350-353), and
+ * J[p][a] = its exitance without the ballistic term (nx*ny x nparam).  For n != 1 that is n
+ * times the series a source of -c_m phi would give (c_m = c/n, the speed in the medium).
  * The reference approximates dA/dmu by -A chi A and ignores the ballistic and source terms,
  * so this is not the exact derivative rte_exitance_vjp gives; it reproduces the reference's
  * gradients (the mu_s region applied as intended: the reference indexes it with a stale

@@ -764,12 +764,13 @@ int orc_solve_images(const orc_grid *g, const orc_optics *o, const orc_beam *b, 
  * restricted to the region and scaled by -C (mu_a) or by -C + g^l (mu_s,
  * PropScatmu1, GPU package/src/dist.cu:675-686); Neumann(src) and the
  * transmitted image without the ballistic term (generate_trans_image(out3, 0)).
- * C is the speed in the medium of the reference's units (C = 1 with n = 1 in
- * FisherInfo_IO/src/rte.h:15, whose scatter carries no speed factor); with this
- * restatement's c_m in A (1/c_m) and D (c_m) the same derivative has the source
- * c_m (-1) phi|_R and c_m (g^l - 1) phi|_R.  The region is applied as
- * intended in both branches (the reference's mu_s branch indexes the region
- * with a stale voxel index, compute_grad_rte.cu:357-366). */
+ * C is the vacuum speed of light (#define C 1, FisherInfo_IO/src/rte.h:15; the
+ * optics' c here, in whatever units the optics use), so the source is
+ * -c phi|_R (mu_a) and (g^l - 1) c phi|_R (mu_s: PropScatmu1's g^l times the
+ * field, plus -C times it), phi in the reference's units (A carrying 1/c_m).
+ * For n != 1 this is n times the source -c_m phi|_R, c_m = c/n.  The region is
+ * applied as intended in both branches (the reference's mu_s branch indexes the
+ * region with a stale voxel index, compute_grad_rte.cu:357-366). */
 int orc_ref_derivative(const orc_grid *g, const orc_optics *o, const orc_beam *b, const orc_params *p,
                        const int *region, int which, double *dimg, orc_stats *st)
 {
@@ -780,7 +781,7 @@ int orc_ref_derivative(const orc_grid *g, const orc_optics *o, const orc_beam *b
     if (rc == 0) {
         for (int a = 0; a < K; ++a) {
             const int l = (int)sqrt((double)a + 0.5);
-            const double f = (o->c / o->n) * (which == 0 ? -1.0 : pow(o->g, (double)l) - 1.0);
+            const double f = o->c * (which == 0 ? -1.0 : pow(o->g, (double)l) - 1.0);
             for (int v = 0; v < N; ++v) phi[(size_t)a * N + v] = region[v] ? f * phi[(size_t)a * N + v] : 0.0;
         }
         rc = neumann_from(g, o, p, phi, NULL, dimg, NULL, NULL, st);

@@ -72,6 +72,46 @@ def test_neumann_errors(ctx):
     m.close()
 
 
+@pytest.mark.parametrize("prec,tol", [(rte.PREC_FP64, 1e-12), (rte.PREC_MIXED, 1e-6)])
+@pytest.mark.parametrize("path", [rte.PATH_CACHED, rte.PATH_MARCH])
+def test_neumann_reference_layout(ctx, prec, tol, path):
+    """rte_neumann_ref (the reference's Neumann(src, out, flag) on its complex,
+    voxel-major field) against rte_neumann on the same first term in the real
+    basis (flag 1: same term count, the field mapped by to_reference_layout) and
+    against rte_propagate (flag 0: out <- A out); fields that are not a real
+    radiance, and flag 1 with out not preset to src, are rejected."""
+    shape, L = (6, 5, 4), 3
+    bounds = bounds_for(shape)
+    ma, ms = hetero_maps(shape, seed=23, mua=0.02, mus=1.2)
+    K, N = (L + 1) ** 2, int(np.prod(shape))
+    rng = np.random.default_rng(9)
+    src = rng.random((K, N)) * np.exp(-np.arange(K))[:, None] - 0.05 * rng.random((K, N))
+    csrc = rte.to_reference_layout(src, L)
+    m = ctx.medium(shape, bounds, ma, ms, 0.8, n=1.37)
+    p = rte.make_params(L, precision=prec, stop_rule=rte.STOP_LAMBDA, path_mode=path)
+    print("neumann_ref flag 1", prec, path, flush=True)
+    f, _, _, st = ctx.neumann(m, src, p)
+    out, st2 = ctx.neumann_ref(m, csrc, p, flag=1)
+    assert st2["terms"] == st["terms"]
+    ref = rte.to_reference_layout(f, L)
+    assert np.max(np.abs(out - ref)) < tol * np.max(np.abs(ref))
+    assert rel_max(rte.from_reference_layout(out, L), f) < tol
+    print("neumann_ref flag 0", prec, path, flush=True)
+    a0, _ = ctx.neumann_ref(m, None, p, flag=0, out=csrc)
+    aw = ctx.propagate(shape, bounds, ma, ms, 0.8, src, L, precision=prec, path_mode=path, n=1.37)
+    ref0 = rte.to_reference_layout(aw, L)
+    assert np.max(np.abs(a0 - ref0)) < tol * np.max(np.abs(ref0))
+    bad = csrc.copy()
+    bad[3, 1] += 0.5j          # Im c_10 != 0: not a real radiance
+    with pytest.raises(rte.RteError) as ei:
+        ctx.neumann_ref(m, bad, p, flag=1)
+    assert ei.value.code == rte.E_ARG
+    with pytest.raises(rte.RteError) as ei:
+        ctx.neumann_ref(m, csrc, p, flag=1, out=2.0 * csrc)
+    assert ei.value.code == rte.E_ARG
+    m.close()
+
+
 @pytest.mark.parametrize("prec,tol", [(rte.PREC_FP64, 1e-8), (rte.PREC_MIXED, 1e-4)])
 def test_fim_scan_against_oracle(ctx, prec, tol):
     n, a, L = 8, 0.4, 3
@@ -91,7 +131,7 @@ def test_fim_scan_against_oracle(ctx, prec, tol):
     for i, z in enumerate(depths):
         for j, r in enumerate(radii):
             inside = fim.region_mask(shape, bounds, (n // 2, n // 2, z), r)
-            dw, _, _, _ = P.neumann(np.where(inside[None, :], -(c / nr) * w, 0.0), L)
+            dw, _, _, _ = P.neumann(np.where(inside[None, :], -c * w, 0.0), L)
             dg = snr.fim_top_image(dw, shape, bounds, L, c=c, n=nr, mag=beam[3])
             F = fim.fisher_top(g0, dg)
             assert abs(res["fisher"][i, j] - F) <= tol * abs(F), (z, r)

@@ -149,10 +149,13 @@ def test_batch_equals_alone(ctx):
         m.close()
 
 
+DEFAULT_GROUP = 3  # rte_ctx_set_group_size's default
+
+
 def test_beams_sharing_a_medium_equal_alone(ctx):
-    """Seven beams on one medium, propagated as groups of 5 and 2 (group size 5)
-    or 2, 2, 2 and 1 (group size 2) right-hand sides, give the same bits as each
-    beam solved alone."""
+    """Seven beams on one medium, propagated as groups of 5 and 2 (group size 5),
+    3, 3 and 1 (group size 3, the default) or 2, 2, 2 and 1 (group size 2)
+    right-hand sides, give the same bits as each beam solved alone."""
     shape, L = (12, 12, 12), 5
     bounds = bounds_for(shape)
     m = ctx.medium(shape, bounds, *hetero_maps(shape, seed=31), 0.9)
@@ -161,20 +164,21 @@ def test_beams_sharing_a_medium_equal_alone(ctx):
     p = rte.make_params(L, precision=rte.PREC_MIXED, stop_rule=rte.STOP_DELTA, tol=1e-6, max_terms=20)
     alone = [ctx.solve_media([m], [b], p) for b in beams]
     try:
-        for gsize in (5, 2):
+        for gsize in (5, DEFAULT_GROUP, 2):
             ctx.set_group_size(gsize)
             batch, st = ctx.solve_media([m] * 7, beams, p)
             for i in range(7):
                 assert alone[i][1][0]["terms"] == st[i]["terms"]
                 assert np.array_equal(alone[i][0][0], batch[i]), (gsize, i)
     finally:
-        ctx.set_group_size(2)
+        ctx.set_group_size(DEFAULT_GROUP)
     m.close()
 
 
 def test_solves_on_different_media_equal_alone(ctx):
-    """Five media with one beam each in one batch, with group sizes 2 and 1:
-    every result has the same bits as the solve alone."""
+    """Five media with one beam each in one batch, at the default group size
+    (pairs across media) and group size 1: every result has the same bits as
+    the solve alone."""
     shape, L = (12, 12, 12), 5
     bounds = bounds_for(shape)
     mm = [ctx.medium(shape, bounds, *hetero_maps(shape, seed=40 + k), 0.85) for k in range(5)]
@@ -187,7 +191,7 @@ def test_solves_on_different_media_equal_alone(ctx):
         ctx.set_group_size(1)
         unpaired, _ = ctx.solve_media(mm, beams, p)
     finally:
-        ctx.set_group_size(2)
+        ctx.set_group_size(DEFAULT_GROUP)
     for k in range(5):
         assert np.array_equal(batch[k], alone[k]), k
         assert np.array_equal(unpaired[k], alone[k]), k

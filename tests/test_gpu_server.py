@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from doi_amd import media, rte
-from tests.helpers import bounds_for, hetero_maps
+from tests.helpers import bounds_for, hetero_maps, rel_max
 
 pytestmark = pytest.mark.gpu
 
@@ -131,19 +131,51 @@ def test_admission_budget_cancel_and_back_pressure():
         sctx.close()
 
 
-def test_served_stream_sample_matches_alone(ctx):
-    """A small seeded sample of the mixed serving stream (grid edges 8-32,
-    L 3/5, random g, mu and beam): served results equal the solves run alone."""
-    reqs = media.serving_requests(12, seed=4, edges=(8, 12, 16), rate=0.0)
+def _stream_sample():
+    """The first request of every (grid edge, L) class of the seeded serving
+    stream (2000 requests, edges 8-32, L 3/5): ten requests, one per bucket,
+    including the 24^3 and 32^3 ones that dominate the stream's device time."""
+    reqs = media.serving_requests(2000, seed=4, rate=0.0)
+    first = {}
+    for r in reqs:
+        first.setdefault((r["shape"][0], r["L"]), r)
+    assert sorted(first) == [(e, L) for e in (8, 12, 16, 24, 32) for L in (3, 5)]
+    return [first[k] for k in sorted(first)]
+
+
+def test_served_stream_sample_matches_alone_and_fp64(ctx):
+    """Every (grid, L) bucket of the serving stream, 8^3 to 32^3 at L 3 and 5,
+    submitted together and then staggered: each served result has the bits of
+    the same solve run alone (rte_solve) and is within 1e-4 of the all-FP64
+    solve with the same term count."""
+    reqs = _stream_sample()
     sctx = rte.Context(0)
     try:
         with rte.Server(sctx) as srv:
+            print("serving sample: all at once", flush=True)
             ids = [srv.submit(r["shape"], r["bounds"], r["mu_a"], r["mu_s"], r["g"], r["beam"], r["params"])
                    for r in reqs]
-            served = [srv.wait(i, 60000.0) for i in ids]
+            served = [srv.wait(i, 120000.0) for i in ids]
+            print("serving sample: staggered, largest first", flush=True)
+            ids = []
+            for k, r in enumerate(reversed(reqs)):
+                ids.append(srv.submit(r["shape"], r["bounds"], r["mu_a"], r["mu_s"], r["g"], r["beam"],
+                                      r["params"]))
+                time.sleep(0.002 * (k % 3))
+            staggered = list(reversed([srv.wait(i, 120000.0) for i in ids]))
+            info = srv.info()
+        assert info["failed"] == 0 and info["worker_status"] == 0 and info["buckets"] >= 10
     finally:
         sctx.close()
-    for r, s in zip(reqs, served):
+    for r, s1, s2 in zip(reqs, served, staggered):
+        n, L = r["shape"][0], r["L"]
+        print("alone and FP64: %d^3 L=%d" % (n, L), flush=True)
         img, st = ctx.solve(r["shape"], r["bounds"], r["mu_a"], r["mu_s"], r["g"], r["beam"], r["params"])
-        assert s[0] == rte.REQ_DONE and s[2]["terms"] == st["terms"]
-        assert np.array_equal(s[1], img)
+        for s in (s1, s2):
+            assert s[0] == rte.REQ_DONE and s[2]["terms"] == st["terms"], (n, L)
+            assert np.array_equal(s[1], img), (n, L)
+        p64 = rte.make_params(L, precision=rte.PREC_FP64, stop_rule=rte.STOP_DELTA, tol=0.0,
+                              max_terms=st["terms"])
+        ref, st64 = ctx.solve(r["shape"], r["bounds"], r["mu_a"], r["mu_s"], r["g"], r["beam"], p64)
+        assert st64["terms"] == st["terms"]
+        assert rel_max(img, ref) < 1e-4, (n, L, rel_max(img, ref))

@@ -331,28 +331,95 @@ def test_lognormal_16cube_fixture_reproduces():
     assert len(lam["exitance"]) == 16 and len(lam["exitance"][0]) == 16
 
 
-def _ref_derivative_by_operators(P, beam, L, region, which, n):
-    """The reference's forward-mode derivative composed from the oracle's
-    separately pinned operators: phi = A out of the forward solve, the
-    derivative source c_m f_l phi on the region (f_l = -1 for mu_a, g^l - 1
-    for mu_s), n Neumann terms, exitance without the ballistic term."""
-    K = (L + 1) ** 2
-    fwd = P.solve(beam, L, stop_rule=O.STOP_DELTA, tol=0.0, max_terms=n, field=True)
-    phi = fwd["w"]
-    c_m = P.optics.c / P.optics.n
-    ls = np.floor(np.sqrt(np.arange(K) + 0.5)).astype(int)
-    f = -np.ones(K) if which == 0 else P.optics.g ** ls - 1.0
-    W = c_m * f[:, None] * phi * (np.ravel(region) != 0)[None, :]
-    E = np.zeros(P.shape[0] * P.shape[1])
+def _series_exitance(P, s0, L, n, delta=None, eps=0.0):
+    """Exitance without the ballistic term of the forward series with the
+    scatter D replaced by D + eps Delta, from the oracle's separately pinned
+    source, propagation, scatter and exitance:  W_0 = s0, W_{q+1} = (D + eps
+    Delta) A W_q (n terms after W_0), E = e(A sum_q W_q)."""
+    W = s0.copy()
+    out = s0.copy()
     for _ in range(n):
         AW = P.propagate(W, L)
-        E += P.exitance(AW, L)
         W = P.scatter(AW, L)
-    return E.reshape(P.shape[1], P.shape[0])
+        if delta is not None:
+            W = W + eps * delta(AW)
+        out += W
+    return P.exitance(P.propagate(out, L), L).reshape(P.shape[1], P.shape[0])
+
+
+def _reference_factor(P, L, which):
+    """Per-coefficient factor of the reference's derivative source on its field
+    out2 (reference units): -1.0*C for mu_a (compute_grad_rte.cu:292) and
+    g^l out2 - C out2 for mu_s (PropScatmu1, GPU package/src/dist.cu:675-686, plus
+    scale_dist(out3, -1.0*C), compute_grad_rte.cu:350-353), C the vacuum speed
+    (FisherInfo_IO/src/rte.h:15), the optics' c here."""
+    K = (L + 1) ** 2
+    ls = np.floor(np.sqrt(np.arange(K) + 0.5)).astype(int)
+    C = P.optics.c
+    return -C * np.ones(K) if which == 0 else P.optics.g ** ls * C - C
 
 
 @pytest.mark.parametrize("which", [0, 1])
-def test_reference_derivative_composition_and_additivity(which):
+@pytest.mark.parametrize("nref", [1.0, 1.37])
+def test_reference_derivative_is_the_scatter_perturbation(which, nref):
+    """The reference's forward-mode derivative dE = e(A sum_j (DA)^j Delta phi),
+    phi = A sum_q (DA)^q s0, is the derivative in eps of the forward series
+    with D replaced by D + eps Delta, Delta = the reference's factor on the
+    region (A and s0 held fixed).  Central differences in eps of that series,
+    built from the pinned operators, must give orc_ref_derivative's image: a
+    wrong field (phi vs sum W), a missing A, a dropped region or a wrong
+    factor all fail.  The series converge to 1e-15 within the term count."""
+    shape, L, g = (4, 4, 5), 2, 0.75
+    bounds = bounds_for(shape)
+    ma, ms = hetero_maps(shape, seed=33, mua=0.2, mus=1.5, sigma=0.3)
+    P = O.Problem(shape, bounds, ma, ms, g, n=nref)
+    beam = (0.0, 0.05, 0.1, 1.0)
+    N = int(np.prod(shape))
+    region = np.zeros(N, dtype=np.int32)
+    region[np.random.default_rng(8).choice(N, 9, replace=False)] = 1
+    n = 40
+    d, terms = P.ref_derivative(beam, L, region, which, stop_rule=O.STOP_DELTA, tol=0.0, max_terms=n)
+    assert terms == n
+    f = _reference_factor(P, L, which)
+
+    def delta(AW):
+        return f[:, None] * AW * region[None, :]
+
+    s0, _, _ = P.source(beam, L)
+    # D = c_m g^l mu_s and Delta are both of order c: eps is a relative step
+    eps = 1e-6
+    Ep = _series_exitance(P, s0, L, n, delta, eps)
+    Em = _series_exitance(P, s0, L, n, delta, -eps)
+    fd = (Ep - Em) / (2 * eps)
+    assert np.max(np.abs(d)) > 0
+    assert rel_max(d, fd) < 1e-7
+    # the series are converged: 10 fewer terms change neither
+    d30, _ = P.ref_derivative(beam, L, region, which, stop_rule=O.STOP_DELTA, tol=0.0, max_terms=n - 10)
+    assert rel_max(d30, d) < 1e-12
+
+
+@pytest.mark.parametrize("which", [0, 1])
+def test_reference_derivative_unit_free(which):
+    """With C read as the optics' speed of light, the derivative image does not
+    depend on the unit of c (the exitance does not): c = 1 (the reference's
+    FisherInfo_IO/src/rte.h:15) and c = 2.9979e10 cm/s give the same image at
+    n = 1.37.  A source of -1 out2 instead of -C out2 would scale with 1/c."""
+    shape, L, g = (4, 3, 4), 2, 0.8
+    bounds = bounds_for(shape)
+    ma, ms = hetero_maps(shape, seed=4, sigma=0.2)
+    region = np.zeros(int(np.prod(shape)), dtype=np.int32)
+    region[5:17] = 1
+    beam = (0.0, 0.0, 0.1, 1.0)
+    imgs = []
+    for c in (1.0, 2.9979e10):
+        P = O.Problem(shape, bounds, ma, ms, g, n=1.37, c=c)
+        imgs.append(P.ref_derivative(beam, L, region, which, stop_rule=O.STOP_DELTA, tol=0.0, max_terms=8)[0])
+    assert np.max(np.abs(imgs[0])) > 0
+    assert rel_max(imgs[1], imgs[0]) < 1e-12
+
+
+@pytest.mark.parametrize("which", [0, 1])
+def test_reference_derivative_additivity(which):
     shape, L, g = (5, 4, 4), 2, 0.7
     bounds = bounds_for(shape)
     ma, ms = hetero_maps(shape, seed=21, sigma=0.3)
@@ -367,9 +434,34 @@ def test_reference_derivative_composition_and_additivity(which):
         d[r], terms = P.ref_derivative(beam, L, (lab == r).astype(np.int32), which, stop_rule=O.STOP_DELTA,
                                        tol=0.0, max_terms=n)
         assert terms == n
-        ref = _ref_derivative_by_operators(P, beam, L, lab == r, which, n)
-        assert rel_max(d[r], ref) < 1e-12
     # the derivative source is linear in the region
     u, _ = P.ref_derivative(beam, L, (lab != 1).astype(np.int32), which, stop_rule=O.STOP_DELTA, tol=0.0,
                             max_terms=n)
     assert rel_max(u, d[0] + d[2]) < 1e-12
+
+
+@pytest.mark.parametrize("L", [0, 1, 3, 5])
+def test_reference_layout_map_preserves_the_radiance(L):
+    """doi_amd.rte.to_reference_layout (the map rte_neumann_ref applies on the
+    device): the complex coefficients it gives describe the same radiance as the
+    real-basis ones, sum_a r_a R_a(u) = sum_lm c_lm Y_lm(u), at random directions,
+    with the oracle's complex harmonics (pinned to scipy and closed forms above);
+    they satisfy c_l,-m = (-1)^m conj c_lm, and from_reference_layout inverts it."""
+    from doi_amd import rte
+    K, N = (L + 1) ** 2, 7
+    rng = np.random.default_rng(L + 40)
+    r = rng.standard_normal((K, N))
+    c = rte.to_reference_layout(r, L)
+    assert c.shape == (N, K)
+    for _ in range(5):
+        th, ph = np.arccos(rng.uniform(-1, 1)), rng.uniform(0, 2 * np.pi)
+        Yc = O.ylm_complex(L, th, ph)
+        Yr = O.ylm_real(L, th, ph)
+        f_real = Yr @ r
+        f_cplx = c @ Yc
+        assert np.max(np.abs(f_cplx.imag)) < 1e-12
+        assert np.max(np.abs(f_cplx.real - f_real)) < 1e-12 * max(1.0, np.max(np.abs(f_real)))
+    for l in range(L + 1):
+        for m in range(1, l + 1):
+            assert np.allclose(c[:, l * l + l - m], (-1) ** m * np.conj(c[:, l * l + l + m]), atol=1e-15)
+    assert np.max(np.abs(rte.from_reference_layout(c, L) - r)) < 1e-14

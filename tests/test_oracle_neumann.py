@@ -36,7 +36,7 @@ def test_neumann_carries_the_reference_derivative_and_is_linear():
     region = np.zeros(P.N, dtype=np.int32)
     region[[17, 18, 37]] = 1
     phi = P.solve(beam, L, field=True)["w"]
-    src = np.where(region[None, :] != 0, -(P.optics.c / P.optics.n) * phi, 0.0)
+    src = np.where(region[None, :] != 0, -P.optics.c * phi, 0.0)   # -1.0*C, compute_grad_rte.cu:292
     _, img, _, terms = P.neumann(src, L)
     ref, ref_terms = P.ref_derivative(beam, L, region, 0)
     assert terms == ref_terms
