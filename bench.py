@@ -122,30 +122,73 @@ def reference_cpu_propagation(n=10, L=5):
                 total_s=stamps[-1][0] if stamps else 0.0)
 
 
-def cpu_baseline(shape, bounds, mu_a, mu_s, g, L, terms, budget_s=12.0):
-    """FP64 oracle (C, OpenMP) timed on this host on a sample of destination
-    voxels of one propagation; scaled to solves/s with the GPU's term count."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _port_per_propagation(P, W, L, budget_s, rows=256):
+    """Seconds per full propagation of the FP64 oracle on problem P, timed on a
+    growing sample of destination voxels until budget_s of CPU work."""
+    N = P.N
+    done, el = 0, 0.0
+    while el < budget_s and done < N:
+        n = min(rows, N - done)
+        _, t = P.propagate_range_timed(W, L, done, done + n)   # the pair loop's own time
+        el += t
+        done += n
+        rows *= 2
+    return el * N / done, done, el
+
+
+def cpu_baseline(shape, bounds, mu_a, mu_s, g, L, terms, budget_s=12.0, budget_1core_s=8.0):
+    """FP64 oracle (C, OpenMP) timed on this host's cores, all of them and one,
+    on a sample of destination voxels of one propagation of the workload's first
+    medium; scaled to solves/s with the GPU's term count."""
     from oracle import oracle as O
     N = int(np.prod(shape))
     P = O.Problem(shape, bounds, mu_a, mu_s, g)
     W = np.random.default_rng(0).standard_normal(((L + 1) ** 2, N))
-    t0 = time.perf_counter()
-    P.propagate_range(W, L, 0, 0)          # table set-up only
-    setup = time.perf_counter() - t0
-    rows, done, el = 256, 0, 0.0
-    while el < budget_s and done < N:
-        n = min(rows, N - done)
-        t0 = time.perf_counter()
-        P.propagate_range(W, L, done, done + n)
-        el += time.perf_counter() - t0 - setup
-        done += n
-        rows *= 2
-    per_prop = el * N / done
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    O.set_threads(cores)
+    per_prop, done, el = _port_per_propagation(P, W, L, budget_s)
+    O.set_threads(1)
+    per_prop1, done1, el1 = _port_per_propagation(P, W, L, budget_1core_s, rows=64)
+    O.set_threads(cores)
     return dict(value=1.0 / (per_prop * terms), unit="solves/s", cores=cores, kind="port",
-                sample="FP64 oracle propagation of one 32^3 L=5 medium over %d of %d destination voxels "
-                       "(%.1f s), scaled to a full propagation (%.1f s) x %.1f Neumann terms per solve"
-                       % (done, N, el, per_prop, terms))
+                value_1core=1.0 / (per_prop1 * terms), cpu_model=cpu_model(),
+                sample="FP64 oracle propagation of the workload's first %dx%dx%d L=%d medium over %d of %d "
+                       "destination voxels on %d cores (%.1f s; %d voxels on 1 core, %.1f s), scaled to a full "
+                       "propagation (%.1f s; %.1f s on 1 core) x %.1f Neumann terms per solve"
+                       % (shape + (L, done, N, cores, el, done1, el1, per_prop, per_prop1, terms)))
+
+
+def reference_calibration(L, n=10):
+    """The reference CPU package (oracle/_ref/rte_cpu, built from its sources)
+    against the FP64 oracle on the same uniform n^3 problem and the same cores:
+    seconds per propagation of each and their ratio.  The reference runs
+    uniform media only, so this ratio, not an extrapolation, is what relates
+    the oracle's 32^3 timing to the reference's code."""
+    ref = reference_cpu_propagation(n, L)
+    if ref is None:
+        return None
+    from oracle import oracle as O
+    shape, bounds = (n, n, n), (-0.05 * n, 0.05 * n, -0.05 * n, 0.05 * n, 0.0, 0.1 * n)
+    P = O.Problem(shape, bounds, 0.01, 1.0, 0.9)
+    W = np.random.default_rng(1).standard_normal(((L + 1) ** 2, n ** 3))
+    O.set_threads(ref["cores"])
+    port, _, _ = _port_per_propagation(P, W, L, 4.0)
+    return dict(problem="uniform %d^3, L=%d, the workload's optics" % (n, L), cores=ref["cores"],
+                reference_s_per_propagation=ref["per_prop_s"], port_s_per_propagation=port,
+                reference_over_port=ref["per_prop_s"] / port,
+                note="reference timed between its own progress lines over %d propagations"
+                     % ref["propagations"])
 
 
 def serve(args):
@@ -215,6 +258,16 @@ def serve(args):
         log("plain %d^3 L%d: %.4f s" % (c[0], c[1], per_class["%d^3 L%d" % c]))
     t_plain = sum(per_class["%d^3 L%d" % (r["shape"][0], r["L"])] for r in reqs)
     plain = len(reqs) / t_plain
+    # error of served results against all-FP64 solves with the same term count:
+    # the first request of every (grid, L) class
+    errs = {}
+    for c in classes:
+        k = next(i for i, r in enumerate(reqs) if (r["shape"][0], r["L"]) == c)
+        r, (status, img, stt) = reqs[k], results[k]
+        p64 = rte.make_params(r["L"], precision=rte.PREC_FP64, stop_rule=rte.STOP_DELTA, tol=0.0,
+                              max_terms=stt["terms"])
+        ref, _ = ctx.solve(r["shape"], r["bounds"], r["mu_a"], r["mu_s"], r["g"], r["beam"], p64)
+        errs["%d^3 L%d" % c] = float(np.max(np.abs(img - ref)) / np.max(np.abs(ref)))
     res = {
         "metric": "RTE serving stream: solves/s + p50/p99 latency (mixed grids 8-32^3, L 3/5)",
         "value": served,
@@ -238,6 +291,8 @@ def serve(args):
         "plain_fp32_solves_per_s": plain,
         "served_over_plain": served / plain,
         "plain_s_per_solve_by_class": per_class,
+        "rel_err_vs_fp64": max(errs.values()),
+        "rel_err_vs_fp64_by_class": errs,
     }
     print(json.dumps(res), flush=True)
     ctx.close()
@@ -322,7 +377,7 @@ def frequency_domain(args):
            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
            "terms_mean": terms_mean, "rel_err_vs_fp64": err,
            "phase_range_deg_medium0_beam0_resolved_pixels": [float(np.degrees(ph.min())), float(np.degrees(ph.max()))],
-           "roofline": {"bound": "mfma", "pipe": "valu-fp32", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+           "roofline": {"bound": "valu", "pipe": "valu-fp32", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                         "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
                         "kernel": "k_prop complex group (cached path)", "avg_launch_ms": avg_ms}}
     if not args.no_cpu:
@@ -423,10 +478,11 @@ def main():
         return gradient_and_fisher(args)
     if args.fd:
         return frequency_domain(args)
-    ref_cpu = None
+    ref_cal = None
     if not args.no_cpu:
-        log("reference CPU package (uniform 10^3 L=5 sample)")
-        ref_cpu = reference_cpu_propagation(10, args.L)
+        # before this process touches the GPU
+        log("reference CPU package against the oracle (uniform 10^3 L=5)")
+        ref_cal = reference_calibration(args.L)
     from doi_amd import media, rte
 
     n, L = args.n, args.L
@@ -503,6 +559,18 @@ def main():
         out["plain_fp32_solves_per_s"] = 1.0 / t_plain
         out["plain_rel_err_vs_fp64"] = float(np.max(np.abs(plain_img[0] - ref[0])) / np.max(np.abs(ref[0])))
         out["batched_over_plain"] = value / out["plain_fp32_solves_per_s"]
+        # the same one-at-a-time FP32 solve with cached path integrals (each solve
+        # builds its medium's table, as an independent solve must)
+        pcached = rte.make_params(L, precision=rte.PREC_FP32, stop_rule=rte.STOP_DELTA, tol=1e-6, max_terms=30,
+                                  path_mode=rte.PATH_CACHED)
+        tc = []
+        for b in range(2):
+            mm[0].reset_cache()
+            t0 = time.perf_counter()
+            ctx.solve_media([mm[0]], [beams5[b]], pcached)
+            tc.append(time.perf_counter() - t0)
+        out["plain_cached_fp32_solves_per_s"] = 1.0 / float(np.mean(tc))
+        out["batched_over_plain_cached"] = value / out["plain_cached_fp32_solves_per_s"]
 
     res = {
         "metric": "RTE solves/s + p50/p99 latency at 32^3 L=5, <=1e-4 rel exitance err vs FP64",
@@ -521,10 +589,13 @@ def main():
                                % (args.media, args.beams, n, L),
                    "solves_per_step": nsolve, "grid": n, "L": L, "precision": "mixed",
                    "path": "cached path integrals"},
-        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "kind": "in-batch: from the batch call's start to each solve's final reduction, "
+                               "all %d solves submitted together (request latency under arrivals: --serve)"
+                               % nsolve},
         "terms_mean": terms_mean,
         "voxel_pair_propagations_per_s": pairs_per_s,
-        "roofline": {"bound": "mfma", "pipe": "valu-fp32", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+        "roofline": {"bound": "valu", "pipe": "valu-fp32", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "kernel": "k_prop (cached path)", "avg_launch_ms": avg_ms,
                      "flops_per_launch": flops_per_launch},
@@ -533,23 +604,9 @@ def main():
     res["ktab_ms_total"] = prof["ktab_ms"]
     if not args.no_cpu:
         log("cpu baseline")
-        port = cpu_baseline(shape, bounds, maps[0][0], maps[0][1], g, L, terms_mean)
-        if ref_cpu is not None:
-            # the reference's propagation is O(pairs): scale the sample by the pair count
-            scale = (float(n) ** 3 / float(ref_cpu["n"]) ** 3) ** 2
-            per_prop = ref_cpu["per_prop_s"] * scale
-            res["cpu_baseline"] = dict(
-                value=1.0 / (per_prop * terms_mean), unit="solves/s", cores=ref_cpu["cores"], kind="reference",
-                sample="reference CPU package (CPU package/src, built from its sources) on a uniform %d^3 L=%d "
-                       "medium of the workload's optics: %.3f s per propagation (%d propagations, %.1f s run), "
-                       "scaled by the voxel-pair count (x%.0f) to %d^3 = %.0f s per propagation, x %.1f Neumann "
-                       "terms per solve; its per-pair cost grows with the grid (the displacement table leaves the "
-                       "caches), so the scaling favours the CPU"
-                       % (ref_cpu["n"], ref_cpu["L"], ref_cpu["per_prop_s"], ref_cpu["propagations"],
-                          ref_cpu["total_s"], scale, n, per_prop, terms_mean))
-            res["cpu_baseline_port"] = port
-        else:
-            res["cpu_baseline"] = port
+        res["cpu_baseline"] = cpu_baseline(shape, bounds, maps[0][0], maps[0][1], g, L, terms_mean)
+        if ref_cal is not None:
+            res["cpu_baseline"]["reference_calibration"] = ref_cal
     print(json.dumps(res), flush=True)
     for m in mm:
         m.close()

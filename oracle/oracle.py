@@ -92,8 +92,21 @@ def lib():
         L.orc_bottom_sum_c.restype = C.c_double
         L.orc_solve_fd.argtypes = [C.POINTER(Grid), C.POINTER(Optics), C.POINTER(Beam), C.POINTER(Params),
                                    C.c_double, dp, dp, dp, C.POINTER(Stats)]
+        L.orc_propagate_range_timed.argtypes = [C.POINTER(Grid), C.POINTER(Optics), C.c_int, C.c_double, dp, dp,
+                                                C.c_int, C.c_int, dp]
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_max_threads.restype = C.c_int
         _LIB = L
     return _LIB
+
+
+def set_threads(n):
+    """Threads of the oracle's OpenMP loops (n > 0)."""
+    lib().orc_set_threads(int(n))
+
+
+def max_threads():
+    return int(lib().orc_max_threads())
 
 
 def _p(a):
@@ -205,6 +218,15 @@ class Problem:
         lib().orc_propagate_range(C.byref(self.grid), C.byref(self.optics), L, prop_thresh, _p(W), _p(out),
                                   d_begin, d_end)
         return out
+
+    def propagate_range_timed(self, W, L, d_begin, d_end, prop_thresh=1e30):
+        """(out, seconds of the pair loop alone) of orc_propagate_range."""
+        W = _f64(W)
+        out = np.zeros_like(W)
+        t = C.c_double(0.0)
+        lib().orc_propagate_range_timed(C.byref(self.grid), C.byref(self.optics), L, prop_thresh, _p(W), _p(out),
+                                        d_begin, d_end, C.byref(t))
+        return out, t.value
 
     def scatter(self, W, L):
         W = _f64(W).copy()

@@ -15,6 +15,8 @@
  */
 #include "rte_oracle.h"
 
+#include <omp.h>
+
 #include <complex.h>
 #include <math.h>
 #include <stdlib.h>
@@ -416,8 +418,25 @@ int orc_propagate(const orc_grid *g, const orc_optics *o, int L, double prop_thr
     return orc_propagate_range(g, o, L, prop_thresh, W, out, 0, g->nx * g->ny * g->nz);
 }
 
+static int propagate_range_impl(const orc_grid *g, const orc_optics *o, int L, double prop_thresh,
+                                const double *W, double *out, int d_begin, int d_end, double *loop_s);
+
 int orc_propagate_range(const orc_grid *g, const orc_optics *o, int L, double prop_thresh,
                         const double *W, double *out, int d_begin, int d_end)
+{
+    return propagate_range_impl(g, o, L, prop_thresh, W, out, d_begin, d_end, NULL);
+}
+
+/* as orc_propagate_range; *loop_s receives the wall time of the pair loop alone
+ * (the tables built before it excluded): the CPU baseline's timer */
+int orc_propagate_range_timed(const orc_grid *g, const orc_optics *o, int L, double prop_thresh,
+                              const double *W, double *out, int d_begin, int d_end, double *loop_s)
+{
+    return propagate_range_impl(g, o, L, prop_thresh, W, out, d_begin, d_end, loop_s);
+}
+
+static int propagate_range_impl(const orc_grid *g, const orc_optics *o, int L, double prop_thresh,
+                                const double *W, double *out, int d_begin, int d_end, double *loop_s)
 {
     geo G;
     geo_init(&G, g);
@@ -465,6 +484,7 @@ int orc_propagate_range(const orc_grid *g, const orc_optics *o, int L, double pr
         free(M);
     }
 
+    const double t_loop = omp_get_wtime();
 #pragma omp parallel for schedule(dynamic, 4)
     for (int d = d_begin; d < d_end; ++d) {
         int di = d % G.nx, dj = (d / G.nx) % G.ny, dk = d / (G.nx * G.ny);
@@ -492,6 +512,7 @@ int orc_propagate_range(const orc_grid *g, const orc_optics *o, int L, double pr
             out[(size_t)a * N + d] = (acc[a] + s) * G.dV;
         }
     }
+    if (loop_s) *loop_s = omp_get_wtime() - t_loop;
     free(Ytab); free(rtab); free(S); free(mut);
     geo_free(&G);
     return 0;
@@ -1113,3 +1134,11 @@ int orc_solve_fd(const orc_grid *g, const orc_optics *o, const orc_beam *b, cons
     free(s0); free(ball); free(W); free(AW); free(out); free(img); free(rimg);
     return 0;
 }
+
+/* Threads of the OpenMP loops (the CPU baseline is timed on one core and on all). */
+void orc_set_threads(int n)
+{
+    if (n > 0) omp_set_num_threads(n);
+}
+
+int orc_max_threads(void) { return omp_get_max_threads(); }

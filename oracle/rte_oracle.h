@@ -140,6 +140,13 @@ int orc_solve_fd(const orc_grid *g, const orc_optics *o, const orc_beam *b, cons
 /* bottom-layer sums of the stop rule (complex-basis real parts) */
 double orc_bottom_sum(const orc_grid *g, int L, const double *W);
 
+/* orc_propagate_range that also returns the pair loop's wall time (tables excluded) */
+int orc_propagate_range_timed(const orc_grid *g, const orc_optics *o, int L, double prop_thresh,
+                              const double *W, double *out, int d_begin, int d_end, double *loop_s);
+/* threads of the OpenMP loops */
+void orc_set_threads(int n);
+int orc_max_threads(void);
+
 #ifdef __cplusplus
 }
 #endif

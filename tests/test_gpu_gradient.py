@@ -193,3 +193,38 @@ def test_reference_jacobian_matches_oracle(ctx, nref):
     F64 = Jf.T @ (Jf / E[:, None])
     assert np.max(np.abs(F - F64)) < 2e-5 * np.max(np.abs(F64))
     med.close()
+
+
+def test_vjp_beyond_the_lds_accumulator_28cube(ctx):
+    """At 28^3 (21952 voxels: the FP64 gradient no longer fits one workgroup's
+    160 KB of LDS) the back-projection adds into device memory with global
+    atomics (k_backproject<..., false>); the adjoint still matches central
+    differences of the forward solve at fixed term count, FP64, L = 1."""
+    shape, L = (28, 28, 28), 1
+    assert shape[0] * shape[1] * shape[2] * 8 > 160 * 1024
+    bounds = bounds_for(shape)
+    ma, ms = hetero_maps(shape, seed=28, sigma=0.3)
+    cot = np.random.default_rng(28).standard_normal(shape[0] * shape[1])
+    n = 4
+    p = rte.make_params(L, precision=rte.PREC_FP64, stop_rule=rte.STOP_DELTA, tol=0.0, max_terms=n)
+    beam = (0.05, -0.05, 0.1, 1.0)
+    med = ctx.medium(shape, bounds, ma, ms, G)
+    print("28^3 vjp", flush=True)
+    _, ga, gs, st = ctx.exitance_vjp(med, beam, p, cot)
+    med.close()
+    assert st["terms"] == n
+
+    def gpu_fun(a, s):
+        e, stt = ctx.solve(shape, bounds, a, s, G, beam, p)
+        return float(np.dot(cot, e.ravel()))
+
+    nx = shape[0]
+    # below the beam, in the middle, at the bottom and off to the side
+    vox = [14 + nx * 13 + nx * nx * 3, 14 + nx * 14 + nx * nx * 14, 10 + nx * 15 + nx * nx * 27,
+           3 + nx * 22 + nx * nx * 9]
+    scale = max(np.max(np.abs(ga)), np.max(np.abs(gs)))
+    for v in vox:
+        for which, adj in (("a", ga[v]), ("s", gs[v])):
+            print("28^3 fd", v, which, flush=True)
+            fd_g = _fd(gpu_fun, ma, ms, which, v)
+            assert _close(adj, fd_g, scale), (v, which, adj, fd_g)
