@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "librte.so")
 
-SOURCES = ["rte_kernels.hip", "rte_adjoint.hip", "rte_host.cpp"]
+SOURCES = ["rte_kernels.hip", "rte_adjoint.hip", "rte_noalloc.hip", "rte_host.cpp"]
 
 # packed FP32 instructions are disabled for all device code
 HIPCC_FLAGS = [

@@ -430,6 +430,8 @@ extern "C" int rte_ctx_create(int device, rte_ctx **out)
     auto ctx = std::make_unique<rte_ctx>();
     ctx->device = device;
     HIP_TRY(hipSetDevice(device));
+    // once per device: the address of rte_propagate_noalloc's scratch array
+    HIP_TRY(scratch_init(device));
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreate(&ctx->ev0));
     HIP_TRY(hipEventCreate(&ctx->ev1));
@@ -2022,6 +2024,157 @@ extern "C" int rte_propagate(rte_ctx *ctx, const rte_grid *grid, const rte_optic
     rte_medium_destroy(m);
     g_err = keep;
     return rc;
+}
+
+// ---- rte_propagate without device allocation --------------------------------
+// Buffers are carved out of the device's scratch array in a fixed order, each
+// on a 256-byte boundary.  size() before any device work tells whether the
+// call fits.
+struct ScratchCarve {
+    unsigned char *base;
+    size_t used = 0;
+    explicit ScratchCarve(void *p) : base(static_cast<unsigned char *>(p)) {}
+    template <typename X>
+    X *take(size_t n)
+    {
+        used = (used + 255) & ~size_t(255);
+        X *p = base ? reinterpret_cast<X *>(base + used) : nullptr;
+        used += n * sizeof(X);
+        return p;
+    }
+};
+
+// The same launches as propagate_impl with the marched path integral, every
+// device buffer in the scratch: the self-block moments (and their quadrature
+// inputs), mu_t and mu_s, the field, the chunk partials, A W and the output.
+// Nothing needs to start at zero: k_prop writes every partial, k_combine every
+// value of A W.
+template <typename T, typename TA>
+static int propagate_noalloc_impl(rte_ctx *ctx, const rte_grid &grid, const rte_optics &opt, int L,
+                                  double prop_thresh, const double *W, double *out)
+{
+    const int K = (L + 1) * (L + 1);
+    const GridDev<T> g = make_grid<T>(grid);
+    const int N = g.N;
+    const double dx = (grid.xmax - grid.xmin) / grid.nx, dy = (grid.ymax - grid.ymin) / grid.ny,
+                 dz = (grid.zmax - grid.zmin) / grid.nz;
+    const double thr = prop_thresh > 0 ? prop_thresh : -1.0;
+    const int nch0 = pick_chunks(g.nblk, 1, N);
+    const int clen = (N + nch0 - 1) / nch0;
+    const int nch = (N + clen - 1) / clen;
+    const size_t nmom = (size_t)kNMom * K * K;
+    auto carve = [&](ScratchCarve &c, double *&R, double *&sn, double *&beta, double *&md, float *&mf, T *&mut,
+                     T *&mus, double *&in, T *&w, T *&part, TA *&aw, double *&o, SolveDev<T, TA> *&sv) {
+        R = c.take<double>((size_t)kNOmega * K);
+        sn = c.take<double>(kNOmega);
+        beta = c.take<double>(kNOmega);
+        md = c.take<double>(nmom);
+        mf = c.take<float>(nmom);
+        mut = c.take<T>(N);
+        mus = c.take<T>(N);
+        in = c.take<double>((size_t)K * N);
+        w = c.take<T>((size_t)K * N);
+        part = c.take<T>((size_t)nch * N * K);
+        aw = c.take<TA>((size_t)K * N);
+        o = c.take<double>((size_t)K * N);
+        sv = c.take<SolveDev<T, TA>>(1);
+    };
+    double *dR, *dsn, *dbeta, *dmd, *dIn, *dOut;
+    float *dmf;
+    T *dmut, *dmus, *dW, *dPart;
+    TA *dAW;
+    SolveDev<T, TA> *dSv;
+    {
+        ScratchCarve probe(nullptr);
+        carve(probe, dR, dsn, dbeta, dmd, dmf, dmut, dmus, dIn, dW, dPart, dAW, dOut, dSv);
+        if (probe.used > kScratchBytes)
+            return fail(RTE_E_UNSUPP,
+                        "rte_propagate_noalloc: %d^3-voxel-equivalent grid (N = %d) at L = %d needs %zu bytes of "
+                        "device scratch, more than its %zu; use rte_propagate",
+                        (int)std::cbrt((double)N), N, L, probe.used, kScratchBytes);
+    }
+    void *base = scratch_ptr(ctx->device);
+    if (!base) return fail(RTE_E_DEVICE, "rte_propagate_noalloc: no scratch address for device %d", ctx->device);
+    ScratchCarve c(base);
+    carve(c, dR, dsn, dbeta, dmd, dmf, dmut, dmus, dIn, dW, dPart, dAW, dOut, dSv);
+    hipStream_t st = ctx->stream;
+    // self-block moments, as get_moments computes them
+    std::vector<double> beta, th, ph;
+    self_beta(dx, dy, dz, beta, th, ph);
+    std::vector<double> R((size_t)kNOmega * K), sn(kNOmega);
+    for (int q = 0; q < kNOmega; ++q) {
+        const double s = std::sin(th[q]);
+        real_sh(L, std::cos(ph[q]) * s, std::sin(ph[q]) * s, std::cos(th[q]), &R[(size_t)q * K]);
+        sn[q] = s;
+    }
+    HIP_TRY(hipMemcpyAsync(dR, R.data(), R.size() * 8, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(dsn, sn.data(), sn.size() * 8, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(dbeta, beta.data(), beta.size() * 8, hipMemcpyHostToDevice, st));
+    HIP_TRY(launch_moments(K, dR, dsn, dbeta, dmd, st));
+    HIP_TRY(launch_narrow(nmom, dmd, dmf, st));
+    // the optical maps in the compute precision, as rte_medium_create forms them
+    std::vector<T> mt(N), ms(N);
+    for (int v = 0; v < N; ++v) {
+        mt[v] = T(opt.mu_a[v] + opt.mu_s[v]);
+        ms[v] = T(opt.mu_s[v]);
+    }
+    HIP_TRY(hipMemcpyAsync(dmut, mt.data(), sizeof(T) * N, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(dmus, ms.data(), sizeof(T) * N, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(dIn, W, (size_t)K * N * 8, hipMemcpyHostToDevice, st));
+    const Scales sc = make_scales(L);
+    HIP_TRY(launch_to_voxel_major<T>(K, N, sc, dIn, dW, st));
+    SolveDev<T, TA> s;
+    std::memset(&s, 0, sizeof s);
+    s.W = dW;
+    s.part = dPart;
+    s.mut = dmut;
+    s.mus = dmus;
+    s.g = T(opt.g);
+    HIP_TRY(hipMemcpyAsync(dSv, &s, sizeof s, hipMemcpyHostToDevice, st));
+    PropArgs<T> pa;
+    pa.g = g;
+    pa.nchunk = nch;
+    pa.chunk_len = clen;
+    pa.thr2 = thr > 0 ? T(thr * thr) : T(sizeof(T) == 4 ? 1e30 : 1e300);
+    HIP_TRY(launch_prop<T, TA>(L, RTE_PATH_MARCH, 1, pa, dSv, 1, st));
+    CombArgs<T, TA> ca;
+    std::memset(&ca, 0, sizeof ca);
+    ca.g = g;
+    ca.nchunk = nch;
+    ca.op = 1;
+    ca.mom = sizeof(TA) == 4 ? (const TA *)(const void *)dmf : (const TA *)(const void *)dmd;
+    ca.momT = sizeof(T) == 4 ? (const T *)(const void *)dmf : (const T *)(const void *)dmd;
+    ca.self_scale = TA((M_PI / kAngRes) * (2.0 * M_PI / kAngRes) / (dx * dy * dz));
+    ca.dV = TA(dx * dy * dz);
+    ca.aw_out = dAW;
+    for (int a = 0; a < kMaxK; ++a) ca.nsc[a] = TA(sc.n[a]);
+    HIP_TRY(launch_combine<T, TA>(L, ca, dSv, 1, st));
+    HIP_TRY(launch_to_coef_major<TA>(K, N, unit_scales(), dAW, dOut, st));
+    HIP_TRY(hipMemcpyAsync(out, dOut, (size_t)K * N * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    // the reference's A carries 1/c_m (CPU package/src/Kernels_selfsub.c:948,1046)
+    const double cm = opt.c / opt.n;
+    for (size_t i = 0; i < (size_t)K * N; ++i) out[i] /= cm;
+    return RTE_OK;
+}
+
+extern "C" int rte_propagate_noalloc(rte_ctx *ctx, const rte_grid *grid, const rte_optics *optics, int L,
+                                     double prop_thresh, int precision, int path_mode, const double *W,
+                                     double *out)
+{
+    if (!ctx || !grid || !optics || !W || !out) return fail(RTE_E_ARG, "rte_propagate_noalloc: NULL argument");
+    if (L < 0 || L > RTE_MAX_L) return fail(RTE_E_UNSUPP, "L=%d outside 0..%d", L, RTE_MAX_L);
+    if (path_mode != RTE_PATH_MARCH)
+        return fail(RTE_E_UNSUPP, "rte_propagate_noalloc: only RTE_PATH_MARCH (a cached path table is a "
+                                  "per-medium allocation of N^2 entries)");
+    int rc;
+    if ((rc = check_grid(grid)) || (rc = check_optics(grid, optics)) || (rc = set_device(ctx))) return rc;
+    switch (precision) {
+    case RTE_PREC_FP64: return propagate_noalloc_impl<double, double>(ctx, *grid, *optics, L, prop_thresh, W, out);
+    case RTE_PREC_MIXED: return propagate_noalloc_impl<float, double>(ctx, *grid, *optics, L, prop_thresh, W, out);
+    case RTE_PREC_FP32: return propagate_noalloc_impl<float, float>(ctx, *grid, *optics, L, prop_thresh, W, out);
+    default: return fail(RTE_E_ARG, "unknown precision %d", precision);
+    }
 }
 
 extern "C" int rte_scatter(rte_ctx *ctx, const rte_grid *grid, const rte_optics *optics, int L, double *W)

@@ -16,6 +16,13 @@ constexpr int kBrick = 4;      // destination bricks of 4x4x4 voxels = one wave6
 constexpr int kWave = 64;
 constexpr int kMaxNR = 5;      // right-hand sides (beams on one medium) per propagation wave
 
+// Device scratch of rte_propagate_noalloc (rte_noalloc.hip): a static array per GPU.
+// scratch_init runs once per device with that device current (rte_ctx_create) and
+// looks up the array's device address; scratch_ptr returns it (nullptr before).
+constexpr size_t kScratchBytes = size_t(64) << 20;
+hipError_t scratch_init(int device);
+void *scratch_ptr(int device);
+
 // Geometry in device form.  Voxel v = x + nx*(y + ny*z); destination bricks
 // are numbered bx fastest; slot = brick*64 + lane, lane = lx + 4*(ly + 4*lz).
 template <typename T>
@@ -154,6 +161,7 @@ hipError_t launch_cplx_to_real(int L, int N, double s, const Scales &sc, const d
 // voxel-major TA [N][K] real basis, times s -> the reference's layout
 template <typename TA>
 hipError_t launch_real_to_cplx(int L, int N, double s, const TA *w, double *c, hipStream_t st);
+hipError_t launch_narrow(size_t n, const double *in, float *out, hipStream_t st);
 hipError_t launch_scatter_op(int L, int N, double g, const double *mus, double *W, hipStream_t st);
 hipError_t launch_exitance_op(int L, int nx, int ny, int nz, const double *coefE, const double *w,
                               double *img, hipStream_t st);

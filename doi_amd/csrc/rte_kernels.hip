@@ -782,6 +782,13 @@ __global__ void k_real_to_cplx(int L, int N, double s, const TA *__restrict__ w,
     c[2 * i + 1] = im * s;
 }
 
+// out[i] = float(in[i]) (round to nearest, as the host's conversion)
+__global__ void k_narrow(size_t n, const double *__restrict__ in, float *__restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (float)in[i];
+}
+
 // D on a coefficient-major double field: W_a(v) *= g^l mu_s(v)
 __global__ void k_scatter_op(int L, int N, double g, const double *__restrict__ mus,
                              double *__restrict__ W)
@@ -1063,6 +1070,12 @@ hipError_t launch_real_to_cplx(int L, int N, double s, const TA *w, double *c, h
 {
     const size_t n = (size_t)(L + 1) * (L + 1) * N;
     hipLaunchKernelGGL(k_real_to_cplx<TA>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, L, N, s, w, c);
+    return hipGetLastError();
+}
+
+hipError_t launch_narrow(size_t n, const double *in, float *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_narrow, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, in, out);
     return hipGetLastError();
 }
 

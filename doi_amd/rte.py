@@ -127,6 +127,8 @@ SIGNATURES = {
     "rte_server_info": (C.c_int, [_vp, C.POINTER(ServerStats)]),
     "rte_propagate": (C.c_int, [_vp, C.POINTER(Grid), C.POINTER(Optics), C.c_int, C.c_double,
                                 C.c_int, C.c_int, _dp, _dp]),
+    "rte_propagate_noalloc": (C.c_int, [_vp, C.POINTER(Grid), C.POINTER(Optics), C.c_int, C.c_double, C.c_int,
+                                        C.c_int, _dp, _dp]),
     "rte_scatter": (C.c_int, [_vp, C.POINTER(Grid), C.POINTER(Optics), C.c_int, _dp]),
     "rte_source": (C.c_int, [_vp, C.POINTER(Grid), C.POINTER(Optics), C.POINTER(Beam), C.c_int,
                              _dp, _dp, _ip]),
@@ -430,6 +432,17 @@ class Context:
         out = np.zeros_like(W)
         _check(lib().rte_propagate(self.h, C.byref(m.grid), C.byref(m.optics), L, prop_thresh,
                                    precision, path_mode, _p(W), _p(out)))
+        return out
+
+    def propagate_noalloc(self, shape, bounds, mu_a, mu_s, g, W, L, prop_thresh=0.0, precision=PREC_FP64,
+                          path_mode=PATH_MARCH, n=1.0, c=2.9979e10):
+        """rte_propagate_noalloc: A W with every device buffer in the library's static
+        scratch array (marched path integrals only)."""
+        m = _Maps(shape, bounds, mu_a, mu_s, g, n, c)
+        W = _f64(W)
+        out = np.zeros_like(W)
+        _check(lib().rte_propagate_noalloc(self.h, C.byref(m.grid), C.byref(m.optics), L, prop_thresh,
+                                           precision, path_mode, _p(W), _p(out)))
         return out
 
     def scatter(self, shape, bounds, mu_a, mu_s, g, W, L, n=1.0, c=2.9979e10):

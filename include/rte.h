@@ -332,6 +332,19 @@ int rte_server_info(rte_server *s, rte_server_stats *out);
 int rte_propagate(rte_ctx *ctx, const rte_grid *grid, const rte_optics *optics, int L,
                   double prop_thresh, int precision, int path_mode, const double *W,
                   double *out);
+/* rte_propagate without device allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments and result as
+ * rte_propagate with path_mode RTE_PATH_MARCH (the only mode accepted: a cached path
+ * table is a per-medium allocation of N^2 entries); every device buffer, the
+ * self-block moments and maps included, lives in one static device array of 64 MiB
+ * per GPU, held until the process ends.  Calls whose buffers would not fit return
+ * RTE_E_UNSUPP before any device work (e.g. 12^3 at L = 5 in FP64 fits, 16^3 at L = 5
+ * in FP64 does not).  Calls on one GPU share that array: calls that overlap on one
+ * GPU (two contexts, two threads) race; one context's calls are ordered. */
+int rte_propagate_noalloc(rte_ctx *ctx, const rte_grid *grid, const rte_optics *optics, int L,
+                          double prop_thresh, int precision, int path_mode, const double *W,
+                          double *out);
 /* D: Henyey-Greenstein scatter g^l mu_s (CPU package/src/Kernels_selfsub.c:30-51), in place */
 int rte_scatter(rte_ctx *ctx, const rte_grid *grid, const rte_optics *optics, int L, double *W);
 /* s0 (K*N) and the ballistic image (nx*ny) (CPU package/src/Dist.c:1077-1128, 203-209;
