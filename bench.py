@@ -392,9 +392,10 @@ def frequency_domain(args):
 def gradient_and_fisher(args):
     """Differentiation workload: 24^3 heterogeneous medium, L=5, mixed
     precision; the loss gradient with respect to every voxel's mu_a and mu_s
-    (one forward + one adjoint series) and the Fisher matrix of 64 block
-    parameters on 576 detectors (J by the reference's per-parameter derivative
-    series, Gram on MFMA, FP64 Cholesky)."""
+    (one forward + one adjoint series), its error against the same gradient in
+    FP64, and the Fisher matrix of 64 block parameters on 576 detectors (J by
+    the reference's per-parameter derivative series, and exact by one adjoint
+    series per detector; Gram on MFMA, FP64 Cholesky)."""
     from doi_amd import media, rte
     n, L, g = 24, 5, 0.8
     shape, bounds = (n, n, n), media.cube_bounds(n)
@@ -419,16 +420,33 @@ def gradient_and_fisher(args):
         t0 = time.perf_counter()
         F, Cf, J, _, stf = ctx.fisher(med, beam, pm, part, 64, wrt=0, jacobian=1)
         tf.append(time.perf_counter() - t0)
-        log("gradient %.3f s, Fisher %.3f s" % (tg[-1], tf[-1]))
+        log("gradient %.3f s, Fisher (reference Jacobian) %.3f s" % (tg[-1], tf[-1]))
+    # the same gradient with every step in FP64: the mixed path's error
+    p64 = rte.make_params(L, precision=rte.PREC_FP64, stop_rule=rte.STOP_DELTA, tol=1e-6, max_terms=30)
+    loss64, img64, ga64, gs64, st64 = ctx.gradient(med, beam, p64, target)
+    gscale = max(np.max(np.abs(ga64)), np.max(np.abs(gs64)))
+    grad_err = max(np.max(np.abs(ga - ga64)), np.max(np.abs(gs - gs64))) / gscale
+    # the exact Fisher information: one adjoint series per detector
+    med.reset_cache()
+    t0 = time.perf_counter()
+    Fx, Cx, Jx, _, stx = ctx.fisher(med, beam, pm, part, 64, wrt=0, jacobian=0)
+    tfx = time.perf_counter() - t0
+    log("exact Fisher %.3f s" % tfx)
     ev = np.linalg.eigvalsh(F)
+    evx = np.linalg.eigvalsh(Fx)
     res = {"metric": "gradient and Fisher time at 24^3 L=5", "value": 1.0 / float(np.mean(tg)),
            "unit": "gradients/s", "n_gpus": 1, "steps": len(tg), "warmup": 1,
            "ms_per_step": 1000.0 * float(np.mean(tg)), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "fp32 (fp64 running sums)", "data": "synthetic (seeded lognormal map with an inclusion)",
            "config": {"workload": "24^3, L=5, g=0.8, loss vs second seeded map; Fisher 64 mu_a blocks "
                                   "(6^3 voxels), 576 detectors"},
-           "terms": st["terms"], "gradient_s": float(np.mean(tg)), "fisher_s": float(np.mean(tf)),
-           "fisher_eig_min_max": [float(ev[0]), float(ev[-1])]}
+           "terms": st["terms"], "gradient_s": float(np.mean(tg)),
+           "gradient_rel_err_vs_fp64": float(grad_err), "terms_fp64": st64["terms"],
+           "loss_rel_err_vs_fp64": float(abs(loss - loss64) / abs(loss64)),
+           "fisher_reference_jacobian_s": float(np.mean(tf)),
+           "fisher_reference_jacobian_eig_min_max": [float(ev[0]), float(ev[-1])],
+           "fisher_exact_s": tfx, "fisher_exact_eig_min_max": [float(evx[0]), float(evx[-1])],
+           "jacobian_reference_vs_exact_rel": float(np.linalg.norm(J - Jx) / np.linalg.norm(Jx))}
     print(json.dumps(res), flush=True)
     med.close()
     ctx.close()

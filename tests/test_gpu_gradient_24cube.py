@@ -92,3 +92,43 @@ def test_fisher_24cube_64_parameters_576_detectors(ctx):
         assert np.max(np.abs(F - F64)) < 2e-5 * np.max(np.abs(F64))
         assert np.max(np.abs(Cf @ Cf.T - F)) < 1e-10 * np.max(np.abs(F))
     med.close()
+
+
+def test_exact_fisher_24cube_columns_against_finite_differences(ctx):
+    """The exact Jacobian at full size (reverse mode, one adjoint series per
+    detector, mixed precision): four of its 64 columns (mu_a on 6^3 blocks, the
+    beam's block, a deep one, a bottom one and a far corner) against central
+    differences of FP64 forward solves at the same term count:
+    |J - FD| <= 2e-3 max|FD| per column.  (The reference's forward-mode
+    Jacobian misses this by tens of percent.)  F is formed from it on MFMA."""
+    ma, ms = _maps(3)
+    med = ctx.medium(SHAPE, BOUNDS, ma, ms, G)
+    i, j, k = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    part = ((i // 6) + 4 * (j // 6) + 16 * (k // 6)).transpose(2, 1, 0).ravel()
+    pm = rte.make_params(L, precision=rte.PREC_MIXED, stop_rule=rte.STOP_DELTA, tol=1e-6, max_terms=30)
+    print("24^3 exact Fisher", flush=True)
+    t0 = time.perf_counter()
+    F, Cf, J, img, st = ctx.fisher(med, BEAM, pm, part, 64, wrt=0, jacobian=0)
+    dt = time.perf_counter() - t0
+    med.close()
+    nt = st["terms"]
+    print("24^3 Fisher (exact Jacobian, 576 adjoint series): %.2f s, %d terms" % (dt, nt), flush=True)
+    E = img.ravel()
+    assert J.shape == (576, 64) and np.all(E > 0)
+    F64 = J.T @ (J / E[:, None])
+    assert np.max(np.abs(F - F64)) < 2e-5 * np.max(np.abs(F64))
+    pf = rte.make_params(L, precision=rte.PREC_FP64, stop_rule=rte.STOP_DELTA, tol=0.0, max_terms=nt)
+    for a in (1 + 4 * 1, 1 + 4 * 1 + 16 * 2, 2 + 4 * 2 + 16 * 3, 3 + 4 * 0 + 16 * 1):
+        blk = part == a
+        h = 1e-3 * float(np.mean(ma[blk]))
+        up, dn = ma.copy(), ma.copy()
+        up[blk] += h
+        dn[blk] -= h
+        print("24^3 FD column", a, flush=True)
+        ep, s1 = ctx.solve(SHAPE, BOUNDS, up, ms, G, BEAM, pf)
+        em, s2 = ctx.solve(SHAPE, BOUNDS, dn, ms, G, BEAM, pf)
+        assert s1["terms"] == s2["terms"] == nt
+        fd = (ep - em).ravel() / (2 * h)
+        err = np.max(np.abs(J[:, a] - fd)) / np.max(np.abs(fd))
+        print("column %d: max |J - FD| / max |FD| = %.2e" % (a, err), flush=True)
+        assert err < 2e-3, (a, err)
