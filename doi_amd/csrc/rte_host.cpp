@@ -1887,6 +1887,33 @@ extern "C" int rte_fisher(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, con
                                      exitance, stats);
 }
 
+extern "C" int rte_fisher_inverse(int nparam, const double *chol, double *Finv)
+{
+    if (nparam < 1 || !chol || !Finv) return fail(RTE_E_ARG, "rte_fisher_inverse: bad nparam/chol/Finv");
+    const int n = nparam;
+    for (int i = 0; i < n; ++i)
+        if (!(chol[(size_t)i * n + i] > 0.0) || !std::isfinite(chol[(size_t)i * n + i]))
+            return fail(RTE_E_ARG, "rte_fisher_inverse: pivot %d is not positive", i);
+    // Cinv = C^-1 (lower triangular) by forward substitution, column by column
+    std::vector<double> Ci((size_t)n * n, 0.0);
+    for (int j = 0; j < n; ++j) {
+        Ci[(size_t)j * n + j] = 1.0 / chol[(size_t)j * n + j];
+        for (int i = j + 1; i < n; ++i) {
+            double s = 0.0;
+            for (int k = j; k < i; ++k) s += chol[(size_t)i * n + k] * Ci[(size_t)k * n + j];
+            Ci[(size_t)i * n + j] = -s / chol[(size_t)i * n + i];
+        }
+    }
+    // F^-1 = Cinv^T Cinv, symmetric
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j <= i; ++j) {
+            double s = 0.0;
+            for (int k = i; k < n; ++k) s += Ci[(size_t)k * n + i] * Ci[(size_t)k * n + j];
+            Finv[(size_t)i * n + j] = Finv[(size_t)j * n + i] = s;
+        }
+    return RTE_OK;
+}
+
 extern "C" int rte_ref_jacobian(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte_params *params,
                                 const int *partition, int nparam, int wrt, double *J, double *exitance,
                                 rte_stats *stats)

@@ -115,6 +115,7 @@ SIGNATURES = {
                                _dp, _dp, C.POINTER(Stats)]),
     "rte_fisher": (C.c_int, [_vp, _vp, C.POINTER(Beam), C.POINTER(Params), C.POINTER(GradOpts), _ip, C.c_int,
                              C.c_int, _dp, _dp, _dp, _dp, C.POINTER(Stats)]),
+    "rte_fisher_inverse": (C.c_int, [C.c_int, _dp, _dp]),
     "rte_ref_jacobian": (C.c_int, [_vp, _vp, C.POINTER(Beam), C.POINTER(Params), _ip, C.c_int, C.c_int, _dp, _dp,
                                    C.POINTER(Stats)]),
     "rte_server_create": (C.c_int, [_vp, C.POINTER(ServerOpts), C.POINTER(_vp)]),
@@ -166,6 +167,15 @@ def _p(a):
 
 def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def fisher_inverse(chol):
+    """F^-1 from the lower Cholesky factor rte_fisher returns (rte_fisher_inverse, FP64)."""
+    chol = np.ascontiguousarray(chol, dtype=np.float64)
+    n = chol.shape[0]
+    out = np.zeros((n, n))
+    _check(lib().rte_fisher_inverse(n, _p(chol), _p(out)))
+    return out
 
 
 def to_reference_layout(field, L):

@@ -267,6 +267,13 @@ int rte_fisher(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const rte_para
                const rte_grad_opts *opts, const int *partition, int nparam, int wrt, double *F,
                double *chol, double *J, double *exitance, rte_stats *stats);
 
+/* F^-1 (nparam x nparam, FP64) from rte_fisher's lower Cholesky factor chol (F = C C^T):
+ * F^-1 = C^-T C^-1 by forward and back substitution; its diagonal is the Cramer-Rao
+ * bound on the parameters' variances.  The reference forms F and stops there
+ * (FisherInfo_IO/src/rte_fim.cu:316-331, inverse_rte.cu:301-316).  Host arithmetic only;
+ * RTE_E_ARG for a non-positive pivot. */
+int rte_fisher_inverse(int nparam, const double *chol, double *Finv);
+
 /* The reference's own derivative of the exitance with respect to block parameters
  * (FisherInfo_IO/src/compute_grad_rte.cu:273-301 mu_a, 341-371 mu_s; GPU package/src/
  * inverse_rte.cu:290-433): phi = A sum_q W_q of the forward solve in the reference's units;

@@ -85,3 +85,23 @@ def test_bench_refuses_more_than_one_gpu():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_fisher_inverse_on_the_host():
+    """rte_fisher_inverse (host FP64, no device): F^-1 from the Cholesky factor
+    of a random SPD matrix equals numpy's inverse, and a non-positive pivot is
+    refused."""
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((18, 18))
+    F = A @ A.T + 18 * np.eye(18)
+    Cf = np.linalg.cholesky(F)
+    Fi = rte.fisher_inverse(Cf)
+    assert np.max(np.abs(Fi - np.linalg.inv(F))) < 1e-12 * np.max(np.abs(np.linalg.inv(F)))
+    assert np.array_equal(Fi, Fi.T)
+    bad = Cf.copy()
+    bad[4, 4] = 0.0
+    try:
+        rte.fisher_inverse(bad)
+        assert False, "zero pivot accepted"
+    except rte.RteError as e:
+        assert e.code == rte.E_ARG

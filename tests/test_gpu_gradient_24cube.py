@@ -117,6 +117,11 @@ def test_exact_fisher_24cube_columns_against_finite_differences(ctx):
     assert J.shape == (576, 64) and np.all(E > 0)
     F64 = J.T @ (J / E[:, None])
     assert np.max(np.abs(F - F64)) < 2e-5 * np.max(np.abs(F64))
+    # F^-1 in FP64 from the Cholesky factor (the Cramer-Rao bound's matrix)
+    Fi = rte.fisher_inverse(Cf)
+    assert np.max(np.abs(F @ Fi - np.eye(64))) < 1e-6
+    assert np.all(np.diag(Fi) > 0)
+    print("Cramer-Rao bound on mu_a per block: %.3g .. %.3g cm^-1" % tuple(np.sqrt(np.diag(Fi))[[0, -1]]))
     pf = rte.make_params(L, precision=rte.PREC_FP64, stop_rule=rte.STOP_DELTA, tol=0.0, max_terms=nt)
     for a in (1 + 4 * 1, 1 + 4 * 1 + 16 * 2, 2 + 4 * 2 + 16 * 3, 3 + 4 * 0 + 16 * 1):
         blk = part == a
