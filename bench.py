@@ -298,6 +298,53 @@ def serve(args):
     ctx.close()
 
 
+def served_latency(maps, beams5, n, L, rate, nreq, seed=11):
+    """Request latency at 32^3 L=5: solves of the workload's own media and
+    beams submitted one at a time to the continuous-batching server with
+    seeded Poisson arrivals at `rate` per second; submit-to-result latency of
+    each, served throughput, and the arrival span."""
+    import threading
+    from doi_amd import media, rte
+    bounds = media.cube_bounds(n)
+    prm = rte.make_params(L, precision=rte.PREC_MIXED, stop_rule=rte.STOP_DELTA, tol=1e-6, max_terms=30)
+    st = media.Stream(seed)
+    gaps = -np.log1p(-st.uniform(nreq)) / rate
+    t_arr = np.cumsum(gaps) - gaps[0]
+    pick = [(k % len(maps), (k // len(maps)) % len(beams5)) for k in range(nreq)]
+    ctx = rte.Context(0)
+    srv = rte.Server(ctx)
+    ma, ms = maps[0]
+    srv.wait(srv.submit((n, n, n), bounds, ma, ms, 0.9, beams5[0], prm))      # warm-up of the bucket
+    ids, res = [None] * nreq, [None] * nreq
+    ready = threading.Semaphore(0)
+
+    def collect():
+        for k in range(nreq):
+            ready.acquire()
+            res[k] = srv.wait(ids[k])
+
+    th = threading.Thread(target=collect)
+    th.start()
+    t0 = time.perf_counter()
+    for k, (mi, bi) in enumerate(pick):
+        dt = t0 + t_arr[k] - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        ma, ms = maps[mi]
+        ids[k] = srv.submit((n, n, n), bounds, ma, ms, 0.9, beams5[bi], prm)
+        ready.release()
+    th.join()
+    wall = time.perf_counter() - t0
+    srv.close()
+    ctx.close()
+    lat = np.array([r[2]["latency_ms"] for r in res])
+    assert all(r[0] == rte.REQ_DONE for r in res)
+    return {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)), "max": float(lat.max()),
+            "requests": nreq, "rate_per_s": rate, "served_solves_per_s": nreq / wall,
+            "kind": "request latency: 32^3 L=5 requests of the workload's media and beams, Poisson arrivals "
+                    "at %.0f/s, submit to result through the continuous-batching server" % rate}
+
+
 def cpu_baseline_fd(shape, bounds, mu_a, mu_s, g, L, omega, terms, budget_s=12.0):
     """cpu_baseline for the frequency-domain workload: the oracle's complex
     propagation on a sample of destination voxels, scaled as cpu_baseline."""
@@ -468,6 +515,9 @@ def main():
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--plain-sample", type=int, default=2)
+    ap.add_argument("--serve-rate", type=float, default=8.0,
+                    help="arrival rate of the 32^3 L=5 request-latency measurement (0: skip it)")
+    ap.add_argument("--serve-requests", type=int, default=80)
     ap.add_argument("--gradient", action="store_true", help="time the gradient and Fisher workload instead")
     ap.add_argument("--fd", action="store_true", help="time the frequency-domain (modulated beam) workload")
     ap.add_argument("--omega", type=float, default=2.0 * np.pi * 1.0e8, help="modulation, rad/s (--fd)")
@@ -544,20 +594,24 @@ def main():
     value = nsolve / (ms_per_step / 1000.0)
     terms_mean = float(np.mean([s["terms"] for s in stats]))
 
-    # roofline of the dominant kernel (propagation): algorithmic flops / launch time
-    launches = max(prof["prop_launches"], 1)
-    avg_ms = prof["prop_ms"] / launches
-    flops_per_launch = prof["prop_flops"] / launches
+    # roofline of the dominant kernel, the propagation group of the default size
+    # (k_prop<float, double, L, 0, 3>): its algorithmic flops / its launch time
+    gsz = args.group or DEFAULT_GROUP
+    launches = max(prof["group_launches"][gsz], 1)
+    avg_ms = prof["group_ms"][gsz] / launches
+    flops_per_launch = prof["group_flops"][gsz] / launches
     achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
     pairs_per_s = prof["prop_pairs"] / (prof["prop_ms"] * 1e-3) if prof["prop_ms"] > 0 else 0.0
-
+    all_launches = max(prof["prop_launches"], 1)
+    achieved_all = prof["prop_flops"] / (prof["prop_ms"] * 1e-3) / 1e12 if prof["prop_ms"] > 0 else 0.0
+    K = (L + 1) ** 2
+    npairs = float(n ** 3) * (n ** 3 - 1)
+    # beam-terms a launch processes: the library counts (16 + k 4K) flops per pair of a group of k
+    beam_terms = flops_per_launch / ((16 + gsz * 4 * K) * npairs) * gsz
     traffic = None
-    gsz = args.group or DEFAULT_GROUP
-    full, rem = divmod(args.beams, gsz)
-    if n == 32 and L == 5 and (full == 0 or gsz in TRAFFIC_PER_SOLVE_TERM) and (rem == 0 or rem in TRAFFIC_PER_SOLVE_TERM):
-        # the beams of a medium go in groups of gsz, the remainder in one group
-        per_medium = full * gsz * TRAFFIC_PER_SOLVE_TERM.get(gsz, 0.0) + rem * TRAFFIC_PER_SOLVE_TERM.get(rem, 0.0)
-        traffic = per_medium * args.media * terms_mean / launches * args.steps
+    if n == 32 and L == 5 and gsz in TRAFFIC_PER_SOLVE_TERM:
+        traffic = TRAFFIC_PER_SOLVE_TERM[gsz] * beam_terms
+
     out = {}
     # error against an all-FP64 solve of medium 0's beams (same term cap)
     if not args.no_check:
@@ -615,11 +669,22 @@ def main():
         "voxel_pair_propagations_per_s": pairs_per_s,
         "roofline": {"bound": "valu", "pipe": "valu-fp32", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
-                     "kernel": "k_prop (cached path)", "avg_launch_ms": avg_ms,
-                     "flops_per_launch": flops_per_launch},
+                     "kernel": "k_prop<float, double, %d, 0, %d> (groups of %d beams, cached path)" % (L, gsz, gsz),
+                     "launches": launches, "avg_launch_ms": avg_ms, "flops_per_launch": flops_per_launch,
+                     "beam_terms_per_launch": beam_terms,
+                     "all_propagation_launches": {"launches": all_launches,
+                                                  "avg_launch_ms": prof["prop_ms"] / all_launches,
+                                                  "achieved": achieved_all,
+                                                  "frac": achieved_all / PEAK_FP32_TFLOPS}},
     }
     res.update(out)
     res["ktab_ms_total"] = prof["ktab_ms"]
+    if args.serve_rate > 0:
+        for m in mm:
+            m.close()
+        mm = []
+        log("served latency: %d requests at %.0f/s" % (args.serve_requests, args.serve_rate))
+        res["served_latency_ms"] = served_latency(maps, beams5, n, L, args.serve_rate, args.serve_requests)
     if not args.no_cpu:
         log("cpu baseline")
         res["cpu_baseline"] = cpu_baseline(shape, bounds, maps[0][0], maps[0][1], g, L, terms_mean)

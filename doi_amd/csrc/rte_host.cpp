@@ -300,6 +300,7 @@ struct PendingTiming {
     int kind;  // 0 prop, 1 ktab, 2 other
     hipEvent_t a, b;
     double pairs, flops;
+    int group;  // propagation kernel (LaunchTimer::group)
 };
 
 struct rte_ctx {
@@ -354,7 +355,8 @@ struct LaunchTimer {
     rte_ctx *ctx;
     int kind;
     hipEvent_t a = nullptr;
-    LaunchTimer(rte_ctx *c, int k) : ctx(c), kind(k)
+    int group = -1;  // propagation kernel: group size k (0: pairs across media), -1 other
+    LaunchTimer(rte_ctx *c, int k, int grp = -1) : ctx(c), kind(k), group(grp)
     {
         if (ctx->prof && (a = pool_event(ctx))) (void)hipEventRecord(a, ctx->stream);
     }
@@ -364,7 +366,7 @@ struct LaunchTimer {
         hipEvent_t b = pool_event(ctx);
         if (!b) return;
         (void)hipEventRecord(b, ctx->stream);
-        ctx->pending.push_back(PendingTiming{kind, a, b, pairs, flops});
+        ctx->pending.push_back(PendingTiming{kind, a, b, pairs, flops, group});
         a = nullptr;
     }
 };
@@ -383,6 +385,11 @@ static void drain_timings(rte_ctx *ctx)
             ctx->pc.prop_ms += ms;
             ctx->pc.prop_pairs += p.pairs;
             ctx->pc.prop_flops += p.flops;
+            if (p.group >= 0 && p.group <= kMaxNR) {
+                ctx->pc.group_launches[p.group]++;
+                ctx->pc.group_ms[p.group] += ms;
+                ctx->pc.group_flops[p.group] += p.flops;
+            }
         } else if (p.kind == 1) {
             ctx->pc.ktab_launches++;
             ctx->pc.ktab_ms += ms;
@@ -986,7 +993,7 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
             HIP_TRY(launch_prop_c<T, TA>(L, path, pa, svd, na, st));
             tm.done(cp_pairs * 2 * na, (flops_dir + 2 * flops_rhs) * na);
         } else if (!multi) {
-            LaunchTimer tm(ctx, 0);
+            LaunchTimer tm(ctx, 0, 1);
             HIP_TRY(launch_prop<T, TA>(L, path, 1, pa, svd, na, st));
             tm.done(cp_pairs * na, (flops_dir + flops_rhs) * na);
         } else {
@@ -1027,12 +1034,12 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
                                    hipMemcpyHostToDevice, st));
             for (auto &ln : launches) {
                 const int k = ln.first, ng = (int)by_size[k].size() / k;
-                LaunchTimer tm(ctx, 0);
+                LaunchTimer tm(ctx, 0, k);
                 HIP_TRY(launch_prop<T, TA>(L, path, k, pa, svd_grp + ln.second, ng, st));
                 tm.done(cp_pairs * k * ng, (flops_dir + k * flops_rhs) * ng);
             }
             if (npair) {
-                LaunchTimer tm(ctx, 0);
+                LaunchTimer tm(ctx, 0, 0);
                 HIP_TRY(launch_prop_sep<T, TA>(L, 2, pa, svd_grp + sep_off, (int)npair, st));
                 tm.done(cp_pairs * 2 * npair, (flops_dir + 2 * flops_rhs) * npair);
             }

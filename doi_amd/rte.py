@@ -65,7 +65,8 @@ class Prof(C.Structure):
     _fields_ = [("prop_launches", C.c_longlong), ("prop_ms", C.c_double), ("prop_pairs", C.c_double),
                 ("prop_flops", C.c_double), ("ktab_launches", C.c_longlong), ("ktab_ms", C.c_double),
                 ("ktab_pairs", C.c_double), ("other_ms", C.c_double), ("path_mode", C.c_int),
-                ("L", C.c_int), ("precision", C.c_int)]
+                ("L", C.c_int), ("precision", C.c_int), ("group_launches", C.c_longlong * 6),
+                ("group_ms", C.c_double * 6), ("group_flops", C.c_double * 6)]
 
 
 class ServerOpts(C.Structure):
@@ -278,7 +279,11 @@ class Context:
     def get_profile(self):
         p = Prof()
         _check(lib().rte_ctx_get_profile(self.h, C.byref(p)))
-        return {k: getattr(p, k) for k, _ in Prof._fields_}
+        out = {}
+        for k, _ in Prof._fields_:
+            v = getattr(p, k)
+            out[k] = list(v) if k.startswith("group_") else v
+        return out
 
     def medium(self, shape, bounds, mu_a, mu_s, g, n=1.0, c=2.9979e10):
         return Medium(self, shape, bounds, mu_a, mu_s, g, n, c)

@@ -120,6 +120,12 @@ typedef struct {
     double ktab_pairs;
     double other_ms;         /* combine, bottom, source launches */
     int path_mode, L, precision; /* of the last propagation launch */
+    /* the real propagation launches by kernel: [k] groups of k right-hand sides on one
+     * medium (k = 1..5), [0] pairs of solves on different media; launches, summed device
+     * time and algorithmic flops of each (the complex groups are in the totals only) */
+    long long group_launches[6];
+    double group_ms[6];
+    double group_flops[6];
 } rte_prof;
 
 typedef struct rte_ctx rte_ctx;
