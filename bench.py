@@ -28,9 +28,10 @@ sys.path.insert(0, ROOT)
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 (vector/matrix dense) published peak
 PEAK_HBM_GBS = 8000.0
 # HBM bytes per solve and term of k_prop at 32^3 L=5 (mixed, cached), from rocprofv3
-# FETCH_SIZE + WRITE_SIZE calibrated against a device copy
-# (profiles/k_prop_32cube_L5_4media_hbm_traffic.md): a beam in a group of three, of two, alone
-TRAFFIC_PER_SOLVE_TERM = {3: 1.52e9, 2: 2.19e9, 1: 4.41e9}
+# FETCH_SIZE + WRITE_SIZE calibrated against a device copy: a beam in a group of three
+# and of two at the benchmark's full size (profiles/k_prop_32cube_L5_bench_hbm_traffic.md),
+# alone on 4 media (profiles/k_prop_32cube_L5_4media_hbm_traffic.md)
+TRAFFIC_PER_SOLVE_TERM = {3: 1.567e9, 2: 2.28e9, 1: 4.41e9}
 DEFAULT_GROUP = 3          # the library's default number of beams per propagation group
 
 

@@ -263,61 +263,68 @@ __global__ void k_make_wq(GridDev<T> g, Scales sc, TA gan, const T *mus, const T
 }
 
 // ---------------------------------------------------------------------------
-// Back-projection: gt(v) += scale * sum_pairs w(pair) len_v(pair), the ray of
-// the pair (destination = lane o, source = row u) walked exactly as the path
-// table was built.  Accumulates in LDS when the grid fits, else in global
-// memory.  grid = (ceil(nblk/4), nchunk), block = 256 (4 bricks).
+// Back-projection: gt(v) += scale * sum_pairs w(pair) len_v(pair), each pair's
+// ray walked as the path table was built, accumulated in LDS when the grid fits
+// and in global memory otherwise, over unordered pairs: the ray between two voxel
+// centres crosses the same voxels with the same Siddon lengths both ways (the
+// crossings alpha = (j + 1/2)/n mirror into each other; only zero-length
+// segments at exact ties may fall in different voxels), so the pair weights of
+// (d, s) and (s, d) are added and the ray is marched once.  A wave owns a
+// brick pair (bd, bs), bs <= bd (the path table's tiling), lane = destination
+// d in bd, and walks the 64 sources of bs; on the diagonal tile only the pairs
+// with slot(s) < slot(d).  grid = ceil(ntiles / waves per block), block =
+// 1024 (LDS accumulator) or 256.
 // ---------------------------------------------------------------------------
 template <typename T, typename TA, bool LDS>
-__global__ __launch_bounds__(1024) void k_backproject(GridDev<T> g, int chunk_len, T thr2, const T *pairbuf,
-                                                      TA scale, TA *gt)
+__global__ __launch_bounds__(1024) void k_backproject_sym(GridDev<T> g, int ntiles, T thr2, const T *pairbuf,
+                                                          TA scale, TA *gt)
 {
     extern __shared__ unsigned char smem_raw[];
     TA *sacc = reinterpret_cast<TA *>(smem_raw);
-    const int lane = threadIdx.x & 63;
-    const int blk = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x >> 6);
-    const int chunk = blockIdx.y;
+    const int tid = threadIdx.x, lane = tid & (kWave - 1);
     if constexpr (LDS) {
-        for (int i = threadIdx.x; i < g.N; i += blockDim.x) sacc[i] = TA(0);
+        for (int i = tid; i < g.N; i += blockDim.x) sacc[i] = TA(0);
         __syncthreads();
     }
-    if (blk < g.nblk) {
+    const int tile = blockIdx.x * (blockDim.x / kWave) + tid / kWave;
+    if (tile < ntiles) {
+        int bd = (int)((sqrtf(8.0f * (float)tile + 1.0f) - 1.0f) * 0.5f);
+        while (bd > 0 && bd * (bd + 1) / 2 > tile) --bd;
+        while ((bd + 1) * (bd + 2) / 2 <= tile) ++bd;
+        const int bs = tile - bd * (bd + 1) / 2;
         int di, dj, dk;
-        lane_dest(g, blk, lane, di, dj, dk);
-        const bool valid = di < g.nx && dj < g.ny && dk < g.nz;
+        lane_dest(g, bd, lane, di, dj, dk);
+        const bool dvalid = di < g.nx && dj < g.ny && dk < g.nz;
         const int cdi = min(di, g.nx - 1), cdj = min(dj, g.ny - 1), cdk = min(dk, g.nz - 1);
-        const int slot = blk * kWave + lane;
-        const int s0 = chunk * chunk_len;
-        const int s1 = min(g.N, s0 + chunk_len);
         const int nxy = g.nx * g.ny;
-        int si = s0 % g.nx, sj = (s0 / g.nx) % g.ny, sk = s0 / nxy;
-        for (int s = s0; s < s1; ++s) {
+        const int dslot = bd * kWave + lane;
+        const int dlin = cdi + g.nx * cdj + nxy * cdk;
+        const int bxi = bs % g.bx, byi = (bs / g.bx) % g.by, bzi = bs / (g.bx * g.by);
+        for (int q = 0; q < kWave; ++q) {
+            const int si = bxi * kBrick + (q & 3), sj = byi * kBrick + ((q >> 2) & 3), sk = bzi * kBrick + (q >> 4);
+            if (si >= g.nx || sj >= g.ny || sk >= g.nz) continue;  // wave-uniform
+            const int sslot = bs * kWave + q;
             const int ix = cdi - si, iy = cdj - sj, iz = cdk - sk;
             const T ddx = T(ix) * g.dx, ddy = T(iy) * g.dy, ddz = T(iz) * g.dz;
             const T r2 = ddx * ddx + ddy * ddy + ddz * ddz;
             const bool self = (ix == 0) && (iy == 0) && (iz == 0);
-            const TA w = TA(pairbuf[(size_t)s * g.npad + slot]);
-            if (valid && !self && r2 < thr2 && w != TA(0)) {
-                const TA sw = scale * w;
-                march_visit<T>(s, ix, iy, iz, g.nx, nxy, sqrt(r2), [&](int v, T len) {
-                    if constexpr (LDS)
-                        atomicAdd(&sacc[v], sw * TA(len));
-                    else
-                        atomicAdd(&gt[v], sw * TA(len));
-                });
-            }
-            if (++si == g.nx) {
-                si = 0;
-                if (++sj == g.ny) {
-                    sj = 0;
-                    ++sk;
-                }
-            }
+            const bool once = bs != bd || sslot < dslot;
+            if (!(dvalid && !self && once && r2 < thr2)) continue;
+            const int slin = si + g.nx * sj + nxy * sk;
+            const TA w = TA(pairbuf[(size_t)slin * g.npad + dslot]) + TA(pairbuf[(size_t)dlin * g.npad + sslot]);
+            if (w == TA(0)) continue;
+            const TA sw = scale * w;
+            march_visit<T>(slin, ix, iy, iz, g.nx, nxy, sqrt(r2), [&](int v, T len) {
+                if constexpr (LDS)
+                    atomicAdd(&sacc[v], sw * TA(len));
+                else
+                    atomicAdd(&gt[v], sw * TA(len));
+            });
         }
     }
     if constexpr (LDS) {
         __syncthreads();
-        for (int i = threadIdx.x; i < g.N; i += blockDim.x)
+        for (int i = tid; i < g.N; i += blockDim.x)
             if (sacc[i] != TA(0)) atomicAdd(&gt[i], sacc[i]);
     }
 }
@@ -494,20 +501,24 @@ hipError_t launch_backproject(const GridDev<T> &g, int nchunk, int chunk_len, T 
         lds_max = v;
     }
     const size_t lds = (size_t)g.N * sizeof(TA);
+    const int ntiles = g.nblk * (g.nblk + 1) / 2;
+    (void)nchunk;
+    (void)chunk_len;
     if (lds <= (size_t)lds_max) {
         // a large accumulator leaves room for one block per CU: make it 16 waves
         const int threads = lds > 64 * 1024 ? 1024 : 256;
         if (lds > 64 * 1024) {
-            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_backproject<T, TA, true>),
+            const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_backproject_sym<T, TA, true>),
                                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             if (e != hipSuccess) return e;
         }
-        const int bricks = threads / kWave;
-        dim3 grid((g.nblk + bricks - 1) / bricks, nchunk), block(threads);
-        hipLaunchKernelGGL((k_backproject<T, TA, true>), grid, block, lds, st, g, chunk_len, thr2, pairbuf, scale, gt);
+        const int waves = threads / kWave;
+        hipLaunchKernelGGL((k_backproject_sym<T, TA, true>), dim3((ntiles + waves - 1) / waves), dim3(threads), lds,
+                           st, g, ntiles, thr2, pairbuf, scale, gt);
     } else {
-        dim3 grid((g.nblk + 3) / 4, nchunk), block(256);
-        hipLaunchKernelGGL((k_backproject<T, TA, false>), grid, block, 0, st, g, chunk_len, thr2, pairbuf, scale, gt);
+        constexpr int threads = 256, waves = threads / kWave;
+        hipLaunchKernelGGL((k_backproject_sym<T, TA, false>), dim3((ntiles + waves - 1) / waves), dim3(threads), 0,
+                           st, g, ntiles, thr2, pairbuf, scale, gt);
     }
     return hipGetLastError();
 }

@@ -198,7 +198,7 @@ def test_reference_jacobian_matches_oracle(ctx, nref):
 def test_vjp_beyond_the_lds_accumulator_28cube(ctx):
     """At 28^3 (21952 voxels: the FP64 gradient no longer fits one workgroup's
     160 KB of LDS) the back-projection adds into device memory with global
-    atomics (k_backproject<..., false>); the adjoint still matches central
+    atomics (k_backproject_sym<..., false>); the adjoint still matches central
     differences of the forward solve at fixed term count, FP64, L = 1."""
     shape, L = (28, 28, 28), 1
     assert shape[0] * shape[1] * shape[2] * 8 > 160 * 1024
