@@ -103,23 +103,28 @@ __global__ __launch_bounds__(kWave * kPropWaves) void k_prop(PropArgs<T> A, cons
     for (int s = s0; s < s1; ++s) {
         const T ddx = dxc - T(si) * g.dx, ddy = dyc - T(sj) * g.dy, ddz = dzc - T(sk) * g.dz;
         T r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-        const bool self = (cdi == si) && (cdj == sj) && (cdk == sk);
         T kfs[NT];
         if constexpr (PATH == 0) {
+            // the self pair's table entry is 0 and its direction (0, 0, 0) whatever r is:
+            // clamping r^2 keeps rsqrt finite without testing the indices
+            r2 = max(r2, T(1e-30));
+            // the next source's entry (the last source's again at the end: in bounds, unused)
+            const int sn = (s + 1 < s1) ? s + 1 : s;
 #pragma unroll
             for (int r = 0; r < NT; ++r) {
                 kfs[r] = kf_next[r];
-                if (s + 1 < s1) kf_next[r] = ktab[r][(size_t)(s + 1) * g.npad + slot];
+                kf_next[r] = ktab[r][(size_t)sn * g.npad + slot];
             }
         } else {
+            const bool self = (cdi == si) && (cdj == sj) && (cdk == sk);
             const int ix = cdi - si, iy = cdj - sj, iz = cdk - sk;
             const bool use = valid && !self && (r2 < A.thr2);
             const T r = sqrt(self ? T(1) : r2);
             const T tau = use ? march_tau<T>(mut, s, ix, iy, iz, g.nx, nxy, r) : T(0);
             kfs[0] = use ? dev_exp<T>(-tau) / r2 : T(0);
+            r2 = self ? T(1) : r2;
         }
         const T kf = kfs[0];
-        r2 = self ? T(1) : r2;
         const T rinv = dev_rsqrt<T>(r2);
         // the pair's expanded harmonics, formed once and shared by every right-hand side
         PairRt<T, L> G;
