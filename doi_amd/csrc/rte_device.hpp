@@ -18,6 +18,18 @@ __device__ __forceinline__ float dev_rsqrt<float>(float x) { return __frsqrt_rn(
 template <>
 __device__ __forceinline__ double dev_rsqrt<double>(double x) { return 1.0 / sqrt(x); }
 
+// a * b + c with one rounding.  The propagation's pair arithmetic is written
+// with it wherever a product meets a sum, and compiled without contraction, so
+// every kernel that evaluates a pair (any group size, alone or across media)
+// rounds the same operations the same way: a solve's bits do not depend on the
+// launch it is in, whatever the compiler schedules around them.
+template <typename T>
+__device__ __forceinline__ T fmad(T a, T b, T c);
+template <>
+__device__ __forceinline__ float fmad<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+template <>
+__device__ __forceinline__ double fmad<double>(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <typename T>
 __device__ __forceinline__ T dev_exp(T x);
 template <>
@@ -182,12 +194,13 @@ struct PairRt {
 template <typename T, int L>
 __device__ __forceinline__ void pair_rt(PairRt<T, L> &G, T x, T y, T z)
 {
+#pragma clang fp contract(off)
     if constexpr (L >= 1) {
         G.R[2] = z;
         T p2 = T(1), p1 = z;
 #pragma unroll
         for (int l = 2; l <= L; ++l) {
-            const T p = z * p1 - sh_gamma<T>(l, 0) * p2;
+            const T p = fmad(z, p1, -(sh_gamma<T>(l, 0) * p2));
             G.R[l * l + l] = p;
             p2 = p1;
             p1 = p;
@@ -196,48 +209,68 @@ __device__ __forceinline__ void pair_rt(PairRt<T, L> &G, T x, T y, T z)
 #pragma unroll
         for (int m = 1; m <= L; ++m) {
             if (m > 1) {
-                const T cn = c * x - s * y, sn = s * x + c * y;
+                const T cn = fmad(c, x, -(s * y)), sn = fmad(s, x, c * y);
                 c = cn;
                 s = sn;
             }
             G.R[m * m + m + m] = c;
             G.R[m * m + m - m] = s;
-            if (m + 1 <= L) {
-                T c2 = c, s2 = s, c1 = z * c, s1 = z * s;
-                G.R[(m + 1) * (m + 1) + (m + 1) + m] = c1;
-                G.R[(m + 1) * (m + 1) + (m + 1) - m] = s1;
+            // the Legendre factor Pt_l^m (Pt_m^m = 1, Pt_m+1^m = z), then its
+            // products with C_m and S_m
+            T q2 = T(1), q1 = z;
 #pragma unroll
-                for (int l = m + 2; l <= L; ++l) {
-                    const T cl = z * c1 - sh_gamma<T>(l, m) * c2, sl = z * s1 - sh_gamma<T>(l, m) * s2;
-                    G.R[l * l + l + m] = cl;
-                    G.R[l * l + l - m] = sl;
-                    c2 = c1;
-                    c1 = cl;
-                    s2 = s1;
-                    s1 = sl;
+            for (int l = m + 1; l <= L; ++l) {
+                const T q = (l == m + 1) ? z : fmad(z, q1, -(sh_gamma<T>(l, m) * q2));
+                if (l > m + 1) {
+                    q2 = q1;
+                    q1 = q;
                 }
+                G.R[l * l + l + m] = c * q;
+                G.R[l * l + l - m] = s * q;
             }
         }
     }
 }
 
-// p = sum_a Rt_a w'_a, in two interleaved chains
+// p = sum_a Rt_a w'_a, in two interleaved chains (odd and even a)
 template <typename T, int L>
 __device__ __forceinline__ T pair_dot_rt(const T (&w)[(L + 1) * (L + 1)], const PairRt<T, L> &G)
 {
+#pragma clang fp contract(off)
     constexpr int K = (L + 1) * (L + 1);
     if constexpr (K == 1) {
         return w[0];
     } else if constexpr (K < 3) {
-        return w[0] + G.R[1] * w[1];
+        return fmad(G.R[1], w[1], w[0]);
     } else {
-        T p = G.R[1] * w[1], q = G.R[2] * w[2];
+        T p = fmad(G.R[1], w[1], w[0]), q = G.R[2] * w[2];
 #pragma unroll
         for (int a = 3; a < K; a += 2) {
-            p += G.R[a] * w[a];
-            if (a + 1 < K) q += G.R[a + 1] * w[a + 1];
+            p = fmad(G.R[a], w[a], p);
+            if (a + 1 < K) q = fmad(G.R[a + 1], w[a + 1], q);
         }
-        return (p + w[0]) + q;
+        return p + q;
+    }
+}
+
+// the chains of pair_dot_rt over coefficients [A0, A0 + N) of a row, read from
+// h[a - A0]: a projection taken in parts, each chain in the same order, so that
+// parts covering [0, K) in turn leave p + q equal to pair_dot_rt's value
+template <typename T, int L, int A0, int N>
+__device__ __forceinline__ void pair_dot_part(T &p, T &q, const T *h, const PairRt<T, L> &G)
+{
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int a = A0; a < A0 + N; ++a) {
+        if (a == 0) continue;
+        if (a == 1)
+            p = fmad(G.R[1], h[1 - A0], h[0 - A0]);
+        else if (a == 2)
+            q = G.R[2] * h[2 - A0];
+        else if (a & 1)
+            p = fmad(G.R[a], h[a - A0], p);
+        else
+            q = fmad(G.R[a], h[a - A0], q);
     }
 }
 
@@ -245,9 +278,10 @@ __device__ __forceinline__ T pair_dot_rt(const T (&w)[(L + 1) * (L + 1)], const 
 template <typename T, int L>
 __device__ __forceinline__ void pair_axpy_rt(T (&acc)[(L + 1) * (L + 1)], T t, const PairRt<T, L> &G)
 {
+#pragma clang fp contract(off)
     acc[0] += t;
 #pragma unroll
-    for (int a = 1; a < (L + 1) * (L + 1); ++a) acc[a] += t * G.R[a];
+    for (int a = 1; a < (L + 1) * (L + 1); ++a) acc[a] = fmad(t, G.R[a], acc[a]);
 }
 
 // Walks the same Siddon crossings as march_tau and calls f(v, length) for

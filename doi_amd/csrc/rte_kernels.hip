@@ -45,6 +45,7 @@ constexpr int kPropWaves = 4;
 template <typename T, typename TA, int L, int PATH, int NR, bool CPLX = false, bool SEP = false>
 __global__ __launch_bounds__(kWave * kPropWaves) void k_prop(PropArgs<T> A, const SolveDev<T, TA> *__restrict__ sv)
 {
+#pragma clang fp contract(off)
     static_assert(!CPLX || NR == 2, "a complex field is propagated as its two real halves");
     // SEP: the group's right-hand sides are solves on different media of one grid; they
     // share the pair geometry and each reads its own path table
@@ -101,8 +102,8 @@ __global__ __launch_bounds__(kWave * kPropWaves) void k_prop(PropArgs<T> A, cons
 #pragma unroll
     for (int a = 0; a < K; ++a) w[a] = (s0 < s1) ? Wsrc[0][(size_t)s0 * K + a] : T(0);
     for (int s = s0; s < s1; ++s) {
-        const T ddx = dxc - T(si) * g.dx, ddy = dyc - T(sj) * g.dy, ddz = dzc - T(sk) * g.dz;
-        T r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+        const T ddx = fmad(-T(si), g.dx, dxc), ddy = fmad(-T(sj), g.dy, dyc), ddz = fmad(-T(sk), g.dz, dzc);
+        T r2 = fmad(ddx, ddx, fmad(ddy, ddy, ddz * ddz));
         T kfs[NT];
         if constexpr (PATH == 0) {
             // the self pair's table entry is 0 and its direction (0, 0, 0) whatever r is:
@@ -155,8 +156,8 @@ __global__ __launch_bounds__(kWave * kPropWaves) void k_prop(PropArgs<T> A, cons
 #pragma unroll
             for (int a = 0; a < K; ++a) w[a] = Wsrc[0][(size_t)snext * K + a];
             __builtin_amdgcn_sched_barrier(0);
-            pair_axpy_rt<T, L>(acc[0], kc * p0 + ks * p1, G);
-            pair_axpy_rt<T, L>(acc[1], kc * p1 - ks * p0, G);
+            pair_axpy_rt<T, L>(acc[0], fmad(kc, p0, ks * p1), G);
+            pair_axpy_rt<T, L>(acc[1], fmad(kc, p1, -(ks * p0)), G);
             __builtin_amdgcn_sched_barrier(0);
         } else if constexpr (sizeof(T) == 4 && K <= 36 && NR == 2) {
             // Groups of two: two sets of scalar registers hold W (2 x 36 floats
@@ -183,6 +184,79 @@ __global__ __launch_bounds__(kWave * kPropWaves) void k_prop(PropArgs<T> A, cons
 #pragma unroll
                 for (int a = 0; a < K; ++a) w[a] = wn[a];
             }
+        } else if constexpr (sizeof(T) == 4 && NR == 3 && K % 2 == 0 && K >= 16 && K <= 36) {
+            // Groups of three: W rows in halves, three half-row sets of scalar
+            // registers (w's two halves and c; two whole rows do not fit beside the
+            // loop's other scalars).  Scalar loads return out of order, so the wait
+            // before a side's projection is for every load in flight: the next
+            // side's halves are requested only after that wait (the first term of
+            // the projection reads the row), the first half at once and the second
+            // into this side's first half once the projection has read it, so
+            // they are covered by most of a projection and an axpy (2K - 3 and
+            // 3K/2 FMAs, against K with one row).  The halves rotate through the
+            // three sets with period three sides: a source starts again with its
+            // first side's row in w.  The empty asm statements order each load
+            // after the terms that free its registers, the scheduling barriers
+            // keep the terms after it from moving ahead of the request.
+            constexpr int H = K / 2;
+            T c[H];
+            T p, q, t;
+            cptr<T> wp;
+            // the harmonics complete before the first projection, so the wait for
+            // this source's first row (requested during the last side of the
+            // previous source) comes after them
+            __builtin_amdgcn_sched_barrier(0);
+            // side 0: halves in w[0, H) and w[H, K); c free
+            pair_dot_part<T, L, 0, 3>(p, q, w, G);
+            wp = Wsrc[1] + (size_t)s * K;
+            asm volatile("" : "+v"(p), "+v"(q), "+s"(wp));
+#pragma unroll
+            for (int a = 0; a < H; ++a) c[a] = wp[a];
+            __builtin_amdgcn_sched_barrier(0);
+            pair_dot_part<T, L, 3, H - 3>(p, q, w + 3, G);
+            wp = Wsrc[1] + (size_t)s * K + H;
+            asm volatile("" : "+v"(p), "+v"(q), "+s"(wp));
+#pragma unroll
+            for (int a = 0; a < H; ++a) w[a] = wp[a];
+            __builtin_amdgcn_sched_barrier(0);
+            pair_dot_part<T, L, H, H>(p, q, w + H, G);
+            t = kfs[0] * (p + q);
+            pair_axpy_rt<T, L>(acc[0], t, G);
+            __builtin_amdgcn_sched_barrier(0);
+            // side 1: halves in c and w[0, H); w[H, K) free
+            pair_dot_part<T, L, 0, 3>(p, q, c, G);
+            wp = Wsrc[2] + (size_t)s * K;
+            asm volatile("" : "+v"(p), "+v"(q), "+s"(wp));
+#pragma unroll
+            for (int a = 0; a < H; ++a) w[H + a] = wp[a];
+            __builtin_amdgcn_sched_barrier(0);
+            pair_dot_part<T, L, 3, H - 3>(p, q, c + 3, G);
+            wp = Wsrc[2] + (size_t)s * K + H;
+            asm volatile("" : "+v"(p), "+v"(q), "+s"(wp));
+#pragma unroll
+            for (int a = 0; a < H; ++a) c[a] = wp[a];
+            __builtin_amdgcn_sched_barrier(0);
+            pair_dot_part<T, L, H, H>(p, q, w, G);
+            t = kfs[SEP ? 1 : 0] * (p + q);
+            pair_axpy_rt<T, L>(acc[1], t, G);
+            __builtin_amdgcn_sched_barrier(0);
+            // side 2: halves in w[H, K) and c; w[0, H) free
+            pair_dot_part<T, L, 0, 3>(p, q, w + H, G);
+            wp = Wsrc[0] + (size_t)snext * K;
+            asm volatile("" : "+v"(p), "+v"(q), "+s"(wp));
+#pragma unroll
+            for (int a = 0; a < H; ++a) w[a] = wp[a];
+            __builtin_amdgcn_sched_barrier(0);
+            pair_dot_part<T, L, 3, H - 3>(p, q, w + H + 3, G);
+            wp = Wsrc[0] + (size_t)snext * K + H;
+            asm volatile("" : "+v"(p), "+v"(q), "+s"(wp));
+#pragma unroll
+            for (int a = 0; a < H; ++a) w[H + a] = wp[a];
+            __builtin_amdgcn_sched_barrier(0);
+            pair_dot_part<T, L, H, H>(p, q, c, G);
+            t = kfs[SEP ? 2 : 0] * (p + q);
+            pair_axpy_rt<T, L>(acc[2], t, G);
+            __builtin_amdgcn_sched_barrier(0);
         } else {
             // the harmonics complete before the first projection: interleaved with
             // it, they pulled the wait for the row requested during the previous

@@ -152,11 +152,13 @@ def test_batch_equals_alone(ctx):
 DEFAULT_GROUP = 3  # rte_ctx_set_group_size's default
 
 
-def test_beams_sharing_a_medium_equal_alone(ctx):
+@pytest.mark.parametrize("L", [3, 4, 5])
+def test_beams_sharing_a_medium_equal_alone(ctx, L):
     """Seven beams on one medium, propagated as groups of 5 and 2 (group size 5),
     3, 3 and 1 (group size 3, the default) or 2, 2, 2 and 1 (group size 2)
-    right-hand sides, give the same bits as each beam solved alone."""
-    shape, L = (12, 12, 12), 5
+    right-hand sides, give the same bits as each beam solved alone (L 3 and 5:
+    the group of three reads W rows in halves; L 4: in whole rows)."""
+    shape = (12, 12, 12)
     bounds = bounds_for(shape)
     m = ctx.medium(shape, bounds, *hetero_maps(shape, seed=31), 0.9)
     xs = (np.arange(12) + 0.5) * 0.1 - 0.6
