@@ -24,11 +24,11 @@ __device__ __forceinline__ double dev_rsqrt<double>(double x) { return 1.0 / sqr
 // rounds the same operations the same way: a solve's bits do not depend on the
 // launch it is in, whatever the compiler schedules around them.
 template <typename T>
-__device__ __forceinline__ T fmad(T a, T b, T c);
+__host__ __device__ __forceinline__ T fmad(T a, T b, T c);
 template <>
-__device__ __forceinline__ float fmad<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__host__ __device__ __forceinline__ float fmad<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 template <>
-__device__ __forceinline__ double fmad<double>(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__host__ __device__ __forceinline__ double fmad<double>(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
 template <typename T>
 __device__ __forceinline__ T dev_exp(T x);
@@ -176,36 +176,74 @@ __device__ __forceinline__ void pair_axpy(T (&acc)[(L + 1) * (L + 1)], T t, cons
 }
 
 // ---------------------------------------------------------------------------
-// The same pair in expanded form for the propagation kernel (every group,
-// real or complex): the direction's scaled real harmonics Rt_a(u) are formed
-// once per pair,
-//   Rt_l0 = Pt_l^0,  Rt_l,m = C_m Pt_l^m,  Rt_l,-m = S_m Pt_l^m   (Rt_00 = 1),
-// the products by the Legendre recurrence in l itself (it is linear, so it
-// holds for C_m Pt_l^m with the same gamma):
-//   X_m,m = C_m,  X_m+1,m = z C_m,  X_l,m = z X_l-1,m - gamma_lm X_l-2,m,
-// and every right-hand side then costs one 36-term dot and one axpy of FMAs,
-// with no per-side products C_m t, S_m t or partial-sum set-up.
+// The same pair for the propagation kernel (every group, real or complex), in
+// the basis of the scaled real harmonics with their Legendre factor replaced by
+// a power of z:
+//   Z_l,m = C_m z^(l-m),  Z_l,-m = S_m z^(l-m)  (C_m + i S_m = (x + i y)^m),
+// stored where Rt_l,+-m = C_m Pt_l^m, S_m Pt_l^m sit (a = l^2 + l +- m).  For
+// each m the powers z^0 .. z^(L-m) span the same polynomials as the monic
+// Legendre factors Pt_m^m .. Pt_L^m, so
+//   Rt_l,m = sum_{l' <= l} P^m_l,l'-m Z_l',m,  P^m_l,k the coefficient of z^k
+// in Pt_l^m (ZPowBasis below).  A pair's 35 values cost one product each
+// beyond the powers (x + i y)^m (43 VALU instructions at L = 5 against 55 with
+// the Legendre recurrence); the propagation reads W in this basis,
+// W'_l',m = sum_{l >= l'} P^m_l,l'-m W_l,m (k_to_zpow, before each
+// propagation launch), sums t Z(u) per pair, and k_combine turns the summed
+// partials back, acc_l,m = sum_{l' <= l} P^m_l,l'-m acc'_l',m.  The map is
+// well conditioned (condition number 3.4 at L = 5, 6.8 at L = 7), so the FP32
+// projection rounds about as the Legendre form does.  (A basis of plain
+// monomials x^a y^b z^c would cost 32 instructions, but its map's condition
+// number, 18 at L = 5 and 88 at L = 7, shows in FP32 results.)  Every
+// right-hand side costs one K-term dot and one K-term axpy of FMAs.
 // ---------------------------------------------------------------------------
+
+// M_aj: Rt_a = sum_j M_aj Z_j; nonzero only between entries of the same m and
+// sign, l' <= l, l' = l mod 2.
+template <int L>
+struct ZPowBasis {
+    static constexpr int K = (L + 1) * (L + 1);
+    double m[K][K];
+    constexpr ZPowBasis() : m{}
+    {
+        for (int mm = 0; mm <= L; ++mm) {
+            double P[L + 2][L + 2] = {};  // P[l][k]: coefficient of z^k in Pt_l^mm (one spare row, column)
+            P[mm][0] = 1.0;
+            if (mm + 1 <= L) P[mm + 1][1] = 1.0;
+            for (int l = mm + 2; l <= L; ++l)
+                for (int k = 0; k <= l - mm; ++k)
+                    P[l][k] = (k ? P[l - 1][k - 1] : 0.0) - sh_gamma<double>(l, mm) * P[l - 2][k];
+            for (int sn = 0; sn < (mm ? 2 : 1); ++sn)
+                for (int l = mm; l <= L; ++l)
+                    for (int lp = mm; lp <= l; ++lp)
+                        this->m[l * l + l + (sn ? -mm : mm)][lp * lp + lp + (sn ? -mm : mm)] = P[l][lp - mm];
+        }
+    }
+};
+
+template <int L>
+inline constexpr ZPowBasis<L> kZPow{};
+
 template <typename T, int L>
-struct PairRt {
+struct PairZ {
     T R[(L + 1) * (L + 1)];  // R[0] = 1 is not used
 };
 
+// the pair's Z values: the powers of z, then (x + i y)^m by complex products
+// and their products with the powers
 template <typename T, int L>
-__device__ __forceinline__ void pair_rt(PairRt<T, L> &G, T x, T y, T z)
+__host__ __device__ __forceinline__ void pair_zpow(PairZ<T, L> &G, T x, T y, T z)
 {
 #pragma clang fp contract(off)
+    G.R[0] = T(1);
     if constexpr (L >= 1) {
-        G.R[2] = z;
-        T p2 = T(1), p1 = z;
+        T zp[L + 1];
+        zp[0] = T(1);
+        zp[1] = z;
 #pragma unroll
-        for (int l = 2; l <= L; ++l) {
-            const T p = fmad(z, p1, -(sh_gamma<T>(l, 0) * p2));
-            G.R[l * l + l] = p;
-            p2 = p1;
-            p1 = p;
-        }
-        T c = x, s = y;  // C_1 + i S_1 = x + i y
+        for (int k = 2; k <= L; ++k) zp[k] = zp[k - 1] * z;
+#pragma unroll
+        for (int l = 1; l <= L; ++l) G.R[l * l + l] = zp[l];
+        T c = x, s = y;
 #pragma unroll
         for (int m = 1; m <= L; ++m) {
             if (m > 1) {
@@ -213,28 +251,18 @@ __device__ __forceinline__ void pair_rt(PairRt<T, L> &G, T x, T y, T z)
                 c = cn;
                 s = sn;
             }
-            G.R[m * m + m + m] = c;
-            G.R[m * m + m - m] = s;
-            // the Legendre factor Pt_l^m (Pt_m^m = 1, Pt_m+1^m = z), then its
-            // products with C_m and S_m
-            T q2 = T(1), q1 = z;
 #pragma unroll
-            for (int l = m + 1; l <= L; ++l) {
-                const T q = (l == m + 1) ? z : fmad(z, q1, -(sh_gamma<T>(l, m) * q2));
-                if (l > m + 1) {
-                    q2 = q1;
-                    q1 = q;
-                }
-                G.R[l * l + l + m] = c * q;
-                G.R[l * l + l - m] = s * q;
+            for (int l = m; l <= L; ++l) {
+                G.R[l * l + l + m] = l == m ? c : c * zp[l - m];
+                G.R[l * l + l - m] = l == m ? s : s * zp[l - m];
             }
         }
     }
 }
 
-// p = sum_a Rt_a w'_a, in two interleaved chains (odd and even a)
+// p = sum_j Z_j w'_j, in two interleaved chains (odd and even j)
 template <typename T, int L>
-__device__ __forceinline__ T pair_dot_rt(const T (&w)[(L + 1) * (L + 1)], const PairRt<T, L> &G)
+__device__ __forceinline__ T pair_dot_z(const T (&w)[(L + 1) * (L + 1)], const PairZ<T, L> &G)
 {
 #pragma clang fp contract(off)
     constexpr int K = (L + 1) * (L + 1);
@@ -253,11 +281,11 @@ __device__ __forceinline__ T pair_dot_rt(const T (&w)[(L + 1) * (L + 1)], const 
     }
 }
 
-// the chains of pair_dot_rt over coefficients [A0, A0 + N) of a row, read from
+// the chains of pair_dot_z over coefficients [A0, A0 + N) of a row, read from
 // h[a - A0]: a projection taken in parts, each chain in the same order, so that
-// parts covering [0, K) in turn leave p + q equal to pair_dot_rt's value
+// parts covering [0, K) in turn leave p + q equal to pair_dot_z's value
 template <typename T, int L, int A0, int N>
-__device__ __forceinline__ void pair_dot_part(T &p, T &q, const T *h, const PairRt<T, L> &G)
+__device__ __forceinline__ void pair_dot_z_part(T &p, T &q, const T *h, const PairZ<T, L> &G)
 {
 #pragma clang fp contract(off)
 #pragma unroll
@@ -274,9 +302,9 @@ __device__ __forceinline__ void pair_dot_part(T &p, T &q, const T *h, const Pair
     }
 }
 
-// acc' += t Rt
+// acc' += t Z
 template <typename T, int L>
-__device__ __forceinline__ void pair_axpy_rt(T (&acc)[(L + 1) * (L + 1)], T t, const PairRt<T, L> &G)
+__device__ __forceinline__ void pair_axpy_z(T (&acc)[(L + 1) * (L + 1)], T t, const PairZ<T, L> &G)
 {
 #pragma clang fp contract(off)
     acc[0] += t;

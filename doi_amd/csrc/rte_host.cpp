@@ -314,7 +314,7 @@ struct rte_ctx {
     std::vector<PendingTiming> pending;
     std::map<std::tuple<int, double, double, double>, std::unique_ptr<MomentSet>> moments;
     // solve workspace (grow-only)
-    DevBuf ws_W, ws_part, ws_pix, ws_sv, ws_beams, ws_red;
+    DevBuf ws_W, ws_Wp, ws_part, ws_pix, ws_sv, ws_beams, ws_red;
     std::set<rte_medium *> media;  // live media of this context
     std::vector<char> host_red;
 };
@@ -819,6 +819,7 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
     const int nch = (N + clen - 1) / clen;
     const size_t szW = (size_t)N * K, szPart = (size_t)nch * N * K;
     HIP_TRY(ctx->ws_W.alloc(szW * sizeof(T) * ns));
+    HIP_TRY(ctx->ws_Wp.alloc(szW * sizeof(T) * ns));
     HIP_TRY(ctx->ws_part.alloc(szPart * sizeof(T) * ns));
     HIP_TRY(ctx->ws_pix.alloc((size_t)kPixBufs * P * sizeof(TA) * ns));
     HIP_TRY(ctx->ws_red.alloc((size_t)2 * sizeof(TA) * ns));
@@ -834,6 +835,7 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
             const int q = i * hv + hh;
             SolveDev<T, TA> &s = sv[q];
             s.W = ctx->ws_W.as<T>() + szW * q;
+            s.Wp = ctx->ws_Wp.as<T>() + szW * q;
             s.part = ctx->ws_part.as<T>() + szPart * q;
             s.ktab = sizeof(T) == 4 ? (const T *)m->ktab_f.p : (const T *)m->ktab_d.p;
             s.mut = sizeof(T) == 4 ? (const T *)m->mut_f.p : (const T *)m->mut_d.p;
@@ -1405,7 +1407,7 @@ extern "C" long long rte_solve_footprint(const rte_grid *grid, const rte_params 
     const long long N = g.N;
     const int nch = pick_chunks(g.nblk, 1, g.N);
     long long bytes = N * 8 * 3 + N * 4 * 2;             // maps
-    bytes += N * K * es * (1 + nch);                      // W + partials
+    bytes += N * K * es * (2 + nch);                      // W, W in the monomial basis, partials
     bytes += (long long)grid->nx * grid->ny * kPixBufs * 8 + 64; // pixel buffers
     if (path_mode == RTE_PATH_CACHED) bytes += N * (long long)g.npad * es;
     return bytes;
@@ -1433,7 +1435,7 @@ struct AdjointRun {
     HostSolve h{};
     SolveDev<T, TA> sv{};
     std::vector<TA> E;
-    DevBuf dW, dS0, dPart, dPix, dRed, dSv, dBeam, dCot, dCoefE, dX, dWq, dPair, dGt, dGs, dGa2, dGs2, dLam0;
+    DevBuf dW, dWp, dS0, dPart, dPix, dRed, dSv, dBeam, dCot, dCoefE, dX, dWq, dPair, dGt, dGs, dGa2, dGs2, dLam0;
     std::vector<std::unique_ptr<DevBuf>> ckpt, seg;
     bool backward_ready = false;
 
@@ -1466,6 +1468,7 @@ struct AdjointRun {
         nch = (N + clen - 1) / clen;
         F = (size_t)N * K;
         HIP_TRY(dW.alloc(F * sizeof(T)));
+        HIP_TRY(dWp.alloc(F * sizeof(T)));
         HIP_TRY(dS0.alloc(F * sizeof(T)));
         HIP_TRY(dPart.alloc((size_t)nch * F * sizeof(T)));
         HIP_TRY(dPix.alloc((size_t)kPixBufs * P * sizeof(TA)));
@@ -1486,6 +1489,7 @@ struct AdjointRun {
         bd = BeamDev{beam->pos_x, beam->pos_y, half, beam->mag / cols, m->mua_d.as<double>(), m->mus_d.as<double>()};
         std::memset(&sv, 0, sizeof sv);
         sv.W = dW.as<T>();
+        sv.Wp = dWp.as<T>();
         sv.part = dPart.as<T>();
         sv.ktab = sizeof(T) == 4 ? (const T *)m->ktab_f.p : (const T *)m->ktab_d.p;
         sv.mut = sizeof(T) == 4 ? (const T *)m->mut_f.p : (const T *)m->mut_d.p;
@@ -1981,10 +1985,11 @@ static int propagate_impl(rte_ctx *ctx, rte_medium *m, int L, double prop_thresh
     const int nch0 = pick_chunks(g.nblk, 1, N);
     const int clen = (N + nch0 - 1) / nch0;
     const int nch = (N + clen - 1) / clen;
-    DevBuf dIn, dW, dPart, dAW, dOut, dSv;
+    DevBuf dIn, dW, dWp, dPart, dAW, dOut, dSv;
     const size_t nio = (size_t)K * N * (cplx ? 2 : 1);
     HIP_TRY(dIn.alloc(nio * 8));
     HIP_TRY(dW.alloc((size_t)K * N * sizeof(T)));
+    HIP_TRY(dWp.alloc((size_t)K * N * sizeof(T)));
     HIP_TRY(dPart.alloc((size_t)nch * N * K * sizeof(T)));
     HIP_TRY(dAW.alloc((size_t)K * N * sizeof(TA)));
     HIP_TRY(dOut.alloc(nio * 8));
@@ -1998,6 +2003,7 @@ static int propagate_impl(rte_ctx *ctx, rte_medium *m, int L, double prop_thresh
     SolveDev<T, TA> s;
     std::memset(&s, 0, sizeof s);
     s.W = dW.as<T>();
+    s.Wp = dWp.as<T>();
     s.part = dPart.as<T>();
     s.ktab = sizeof(T) == 4 ? (const T *)m->ktab_f.p : (const T *)m->ktab_d.p;
     s.mut = sizeof(T) == 4 ? (const T *)m->mut_f.p : (const T *)m->mut_d.p;
@@ -2098,7 +2104,7 @@ static int propagate_noalloc_impl(rte_ctx *ctx, const rte_grid &grid, const rte_
     const int nch = (N + clen - 1) / clen;
     const size_t nmom = (size_t)kNMom * K * K;
     auto carve = [&](ScratchCarve &c, double *&R, double *&sn, double *&beta, double *&md, float *&mf, T *&mut,
-                     T *&mus, double *&in, T *&w, T *&part, TA *&aw, double *&o, SolveDev<T, TA> *&sv) {
+                     T *&mus, double *&in, T *&w, T *&wp, T *&part, TA *&aw, double *&o, SolveDev<T, TA> *&sv) {
         R = c.take<double>((size_t)kNOmega * K);
         sn = c.take<double>(kNOmega);
         beta = c.take<double>(kNOmega);
@@ -2108,6 +2114,7 @@ static int propagate_noalloc_impl(rte_ctx *ctx, const rte_grid &grid, const rte_
         mus = c.take<T>(N);
         in = c.take<double>((size_t)K * N);
         w = c.take<T>((size_t)K * N);
+        wp = c.take<T>((size_t)K * N);
         part = c.take<T>((size_t)nch * N * K);
         aw = c.take<TA>((size_t)K * N);
         o = c.take<double>((size_t)K * N);
@@ -2115,12 +2122,12 @@ static int propagate_noalloc_impl(rte_ctx *ctx, const rte_grid &grid, const rte_
     };
     double *dR, *dsn, *dbeta, *dmd, *dIn, *dOut;
     float *dmf;
-    T *dmut, *dmus, *dW, *dPart;
+    T *dmut, *dmus, *dW, *dWp, *dPart;
     TA *dAW;
     SolveDev<T, TA> *dSv;
     {
         ScratchCarve probe(nullptr);
-        carve(probe, dR, dsn, dbeta, dmd, dmf, dmut, dmus, dIn, dW, dPart, dAW, dOut, dSv);
+        carve(probe, dR, dsn, dbeta, dmd, dmf, dmut, dmus, dIn, dW, dWp, dPart, dAW, dOut, dSv);
         if (probe.used > kScratchBytes)
             return fail(RTE_E_UNSUPP,
                         "rte_propagate_noalloc: %d^3-voxel-equivalent grid (N = %d) at L = %d needs %zu bytes of "
@@ -2130,7 +2137,7 @@ static int propagate_noalloc_impl(rte_ctx *ctx, const rte_grid &grid, const rte_
     void *base = scratch_ptr(ctx->device);
     if (!base) return fail(RTE_E_DEVICE, "rte_propagate_noalloc: no scratch address for device %d", ctx->device);
     ScratchCarve c(base);
-    carve(c, dR, dsn, dbeta, dmd, dmf, dmut, dmus, dIn, dW, dPart, dAW, dOut, dSv);
+    carve(c, dR, dsn, dbeta, dmd, dmf, dmut, dmus, dIn, dW, dWp, dPart, dAW, dOut, dSv);
     hipStream_t st = ctx->stream;
     // self-block moments, as get_moments computes them
     std::vector<double> beta, th, ph;
@@ -2160,6 +2167,7 @@ static int propagate_noalloc_impl(rte_ctx *ctx, const rte_grid &grid, const rte_
     SolveDev<T, TA> s;
     std::memset(&s, 0, sizeof s);
     s.W = dW;
+    s.Wp = dWp;
     s.part = dPart;
     s.mut = dmut;
     s.mus = dmus;
@@ -2411,12 +2419,12 @@ struct SlotBase {
     ServReq *r = nullptr;  // null: idle
     int idx = 0;           // row of the bucket's reduction and image arrays
     long long bytes = 0;
-    DevBuf mua, mus, mutd, mutf, musf, ktab, W, part, pix;
+    DevBuf mua, mus, mutd, mutf, musf, ktab, W, Wp, part, pix;
     HostSolve h{};
     bool fresh = false, ran = false;
     void free_memory()
     {
-        for (DevBuf *b : {&mua, &mus, &mutd, &mutf, &musf, &ktab, &W, &part, &pix}) b->release();
+        for (DevBuf *b : {&mua, &mus, &mutd, &mutf, &musf, &ktab, &W, &Wp, &part, &pix}) b->release();
         bytes = 0;
     }
 };
@@ -2570,7 +2578,7 @@ struct Bucket : BucketBase {
         const bool cached = p.path_mode == RTE_PATH_CACHED;
         if (s.mua.alloc(n * 8) || s.mus.alloc(n * 8) || s.mutd.alloc(n * 8) || s.mutf.alloc(n * 4) ||
             s.musf.alloc(n * 4) || (cached && s.ktab.alloc((size_t)N * g.npad * sizeof(T))) ||
-            s.W.alloc(F * sizeof(T)) || s.part.alloc((size_t)nch * F * sizeof(T)) ||
+            s.W.alloc(F * sizeof(T)) || s.Wp.alloc(F * sizeof(T)) || s.part.alloc((size_t)nch * F * sizeof(T)) ||
             s.pix.alloc((size_t)kPixBufs * P * sizeof(TA))) {
             s.free_memory();
             return fail(RTE_E_NOMEM, "server: device memory for a %dx%dx%d L=%d solve", grid.nx, grid.ny, grid.nz, L);
@@ -2597,6 +2605,7 @@ struct Bucket : BucketBase {
         SolveDev<T, TA> &v = sv[s.idx];
         std::memset(&v, 0, sizeof v);
         v.W = s.W.template as<T>();
+        v.Wp = s.Wp.template as<T>();
         v.part = s.part.template as<T>();
         v.ktab = p.path_mode == RTE_PATH_CACHED ? s.ktab.template as<T>() : nullptr;
         v.mut = mut;

@@ -53,6 +53,7 @@ __host__ __device__ constexpr T sh_gamma(int l, int m)
 template <typename T, typename TA>
 struct SolveDev {
     T *W;              // [N][K] current term (input of the propagation)
+    T *Wp;             // [N][K] the same in the propagation's z-power basis, written by each launch
     T *part;           // [nchunk][N][K] partial pair sums
     const T *ktab;     // [N][npad] e^{-tau}/r^2 per (source, destination slot), cached mode
     const T *mut;      // [N] mu_t

@@ -10,8 +10,11 @@
 // Work decomposition: one wave64 owns a 4x4x4 brick of destination voxels (one
 // per lane) and walks a contiguous chunk of source voxels in lockstep, so the
 // source's coefficients W(s) are wave-uniform (scalar loads into SGPRs) and
-// every lane is a valid pair; R(u) is evaluated per lane with the Cartesian
-// recurrences, fused into the dot product and the axpy.  The path integral
+// every lane is a valid pair; R(u) is taken per lane in the basis C_m z^k,
+// S_m z^k (the harmonics with their Legendre factor replaced by powers of z:
+// one product each; W is converted into that basis before the launch and the
+// summed partials back in k_combine), then one dot product and one axpy per
+// right-hand side.  The path integral
 // either comes from a per-medium table (cached mode, written once by
 // k_ktab) or is marched per pair in every term (march mode, the reference's
 // structure, GPU package/src/kernels.cu:49-145).  Partial sums per source
@@ -71,7 +74,7 @@ __global__ __launch_bounds__(kWave * kPropWaves) void k_prop(PropArgs<T> A, cons
     cptr<T> Wsrc[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
-        Wsrc[r] = as_const<T>(grp[r].W);
+        Wsrc[r] = as_const<T>(grp[r].Wp);
 #pragma unroll
         for (int a = 0; a < K; ++a) acc[r][a] = T(0);
     }
@@ -127,9 +130,9 @@ __global__ __launch_bounds__(kWave * kPropWaves) void k_prop(PropArgs<T> A, cons
         }
         const T kf = kfs[0];
         const T rinv = dev_rsqrt<T>(r2);
-        // the pair's expanded harmonics, formed once and shared by every right-hand side
-        PairRt<T, L> G;
-        pair_rt<T, L>(G, ddx * rinv, ddy * rinv, ddz * rinv);
+        // the pair's Z values, formed once and shared by every right-hand side
+        PairZ<T, L> G;
+        pair_zpow<T, L>(G, ddx * rinv, ddy * rinv, ddz * rinv);
         // w holds right-hand side r of source s; the next one (r+1, or 0 of
         // s+1) is requested before r is applied, hiding the scalar-load latency
         const int snext = (s + 1 < s1) ? s + 1 : s;
@@ -139,7 +142,7 @@ __global__ __launch_bounds__(kWave * kPropWaves) void k_prop(PropArgs<T> A, cons
             // One set of scalar registers holds W (see below): the imaginary
             // half is loaded after the real projection, the next source's real
             // half after the imaginary one, covered by the two axpys.
-            const T p0 = pair_dot_rt<T, L>(w, G);
+            const T p0 = pair_dot_z<T, L>(w, G);
 #pragma unroll
             for (int a = 0; a < K; ++a) w[a] = Wsrc[1][(size_t)s * K + a];
             T sn, cs;
@@ -152,12 +155,12 @@ __global__ __launch_bounds__(kWave * kPropWaves) void k_prop(PropArgs<T> A, cons
             }
             const T kc = kf * cs, ks = kf * sn;
             __builtin_amdgcn_sched_barrier(0);
-            const T p1 = pair_dot_rt<T, L>(w, G);
+            const T p1 = pair_dot_z<T, L>(w, G);
 #pragma unroll
             for (int a = 0; a < K; ++a) w[a] = Wsrc[0][(size_t)snext * K + a];
             __builtin_amdgcn_sched_barrier(0);
-            pair_axpy_rt<T, L>(acc[0], fmad(kc, p0, ks * p1), G);
-            pair_axpy_rt<T, L>(acc[1], fmad(kc, p1, -(ks * p0)), G);
+            pair_axpy_z<T, L>(acc[0], fmad(kc, p0, ks * p1), G);
+            pair_axpy_z<T, L>(acc[1], fmad(kc, p1, -(ks * p0)), G);
             __builtin_amdgcn_sched_barrier(0);
         } else if constexpr (sizeof(T) == 4 && K <= 36 && NR == 2) {
             // Groups of two: two sets of scalar registers hold W (2 x 36 floats
@@ -178,8 +181,8 @@ __global__ __launch_bounds__(kWave * kPropWaves) void k_prop(PropArgs<T> A, cons
                 for (int a = 0; a < K; ++a)
                     wn[a] = (r + 1 < NR) ? Wsrc[r + 1][(size_t)s * K + a] : Wsrc[0][(size_t)snext * K + a];
                 __builtin_amdgcn_sched_barrier(0);
-                const T t = kfs[SEP ? r : 0] * pair_dot_rt<T, L>(w, G);
-                pair_axpy_rt<T, L>(acc[r], t, G);
+                const T t = kfs[SEP ? r : 0] * pair_dot_z<T, L>(w, G);
+                pair_axpy_z<T, L>(acc[r], t, G);
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int a = 0; a < K; ++a) w[a] = wn[a];
@@ -207,55 +210,55 @@ __global__ __launch_bounds__(kWave * kPropWaves) void k_prop(PropArgs<T> A, cons
             // previous source) comes after them
             __builtin_amdgcn_sched_barrier(0);
             // side 0: halves in w[0, H) and w[H, K); c free
-            pair_dot_part<T, L, 0, 3>(p, q, w, G);
+            pair_dot_z_part<T, L, 0, 3>(p, q, w, G);
             wp = Wsrc[1] + (size_t)s * K;
             asm volatile("" : "+v"(p), "+v"(q), "+s"(wp));
 #pragma unroll
             for (int a = 0; a < H; ++a) c[a] = wp[a];
             __builtin_amdgcn_sched_barrier(0);
-            pair_dot_part<T, L, 3, H - 3>(p, q, w + 3, G);
+            pair_dot_z_part<T, L, 3, H - 3>(p, q, w + 3, G);
             wp = Wsrc[1] + (size_t)s * K + H;
             asm volatile("" : "+v"(p), "+v"(q), "+s"(wp));
 #pragma unroll
             for (int a = 0; a < H; ++a) w[a] = wp[a];
             __builtin_amdgcn_sched_barrier(0);
-            pair_dot_part<T, L, H, H>(p, q, w + H, G);
+            pair_dot_z_part<T, L, H, H>(p, q, w + H, G);
             t = kfs[0] * (p + q);
-            pair_axpy_rt<T, L>(acc[0], t, G);
+            pair_axpy_z<T, L>(acc[0], t, G);
             __builtin_amdgcn_sched_barrier(0);
             // side 1: halves in c and w[0, H); w[H, K) free
-            pair_dot_part<T, L, 0, 3>(p, q, c, G);
+            pair_dot_z_part<T, L, 0, 3>(p, q, c, G);
             wp = Wsrc[2] + (size_t)s * K;
             asm volatile("" : "+v"(p), "+v"(q), "+s"(wp));
 #pragma unroll
             for (int a = 0; a < H; ++a) w[H + a] = wp[a];
             __builtin_amdgcn_sched_barrier(0);
-            pair_dot_part<T, L, 3, H - 3>(p, q, c + 3, G);
+            pair_dot_z_part<T, L, 3, H - 3>(p, q, c + 3, G);
             wp = Wsrc[2] + (size_t)s * K + H;
             asm volatile("" : "+v"(p), "+v"(q), "+s"(wp));
 #pragma unroll
             for (int a = 0; a < H; ++a) c[a] = wp[a];
             __builtin_amdgcn_sched_barrier(0);
-            pair_dot_part<T, L, H, H>(p, q, w, G);
+            pair_dot_z_part<T, L, H, H>(p, q, w, G);
             t = kfs[SEP ? 1 : 0] * (p + q);
-            pair_axpy_rt<T, L>(acc[1], t, G);
+            pair_axpy_z<T, L>(acc[1], t, G);
             __builtin_amdgcn_sched_barrier(0);
             // side 2: halves in w[H, K) and c; w[0, H) free
-            pair_dot_part<T, L, 0, 3>(p, q, w + H, G);
+            pair_dot_z_part<T, L, 0, 3>(p, q, w + H, G);
             wp = Wsrc[0] + (size_t)snext * K;
             asm volatile("" : "+v"(p), "+v"(q), "+s"(wp));
 #pragma unroll
             for (int a = 0; a < H; ++a) w[a] = wp[a];
             __builtin_amdgcn_sched_barrier(0);
-            pair_dot_part<T, L, 3, H - 3>(p, q, w + H + 3, G);
+            pair_dot_z_part<T, L, 3, H - 3>(p, q, w + H + 3, G);
             wp = Wsrc[0] + (size_t)snext * K + H;
             asm volatile("" : "+v"(p), "+v"(q), "+s"(wp));
 #pragma unroll
             for (int a = 0; a < H; ++a) w[H + a] = wp[a];
             __builtin_amdgcn_sched_barrier(0);
-            pair_dot_part<T, L, H, H>(p, q, c, G);
+            pair_dot_z_part<T, L, H, H>(p, q, c, G);
             t = kfs[SEP ? 2 : 0] * (p + q);
-            pair_axpy_rt<T, L>(acc[2], t, G);
+            pair_axpy_z<T, L>(acc[2], t, G);
             __builtin_amdgcn_sched_barrier(0);
         } else {
             // the harmonics complete before the first projection: interleaved with
@@ -269,12 +272,12 @@ __global__ __launch_bounds__(kWave * kPropWaves) void k_prop(PropArgs<T> A, cons
             // above the axpy (two live sets of W do not fit the SGPR file).
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
-                const T t = kfs[SEP ? r : 0] * pair_dot_rt<T, L>(w, G);
+                const T t = kfs[SEP ? r : 0] * pair_dot_z<T, L>(w, G);
 #pragma unroll
                 for (int a = 0; a < K; ++a)
                     w[a] = (r + 1 < NR) ? Wsrc[r + 1][(size_t)s * K + a] : Wsrc[0][(size_t)snext * K + a];
                 __builtin_amdgcn_sched_barrier(0);
-                pair_axpy_rt<T, L>(acc[r], t, G);
+                pair_axpy_z<T, L>(acc[r], t, G);
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
@@ -287,6 +290,7 @@ __global__ __launch_bounds__(kWave * kPropWaves) void k_prop(PropArgs<T> A, cons
         }
     }
     if (valid) {
+        // partial sums in the Z basis (k_combine turns their chunk sum back)
         const int v = di + g.nx * (dj + g.ny * dk);
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
@@ -294,6 +298,33 @@ __global__ __launch_bounds__(kWave * kPropWaves) void k_prop(PropArgs<T> A, cons
 #pragma unroll
             for (int a = 0; a < K; ++a) out[a] = acc[r][a];
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// A propagation's input in the Z basis: W'_j = sum_a M_aj W_a for every
+// solve of the launch (their SolveDev entries are consecutive), from W into Wp.
+// grid = (ceil(N / 256), nsolve), block = 256: one thread per voxel and solve.
+// ---------------------------------------------------------------------------
+template <typename T, typename TA, int L>
+__global__ __launch_bounds__(256) void k_to_zpow(int N, const SolveDev<T, TA> *__restrict__ sv)
+{
+    constexpr int K = (L + 1) * (L + 1);
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    const SolveDev<T, TA> &s = sv[blockIdx.y];
+    const T *__restrict__ w = s.W + (size_t)v * K;
+    T *__restrict__ o = s.Wp + (size_t)v * K;
+    double x[K];
+#pragma unroll
+    for (int a = 0; a < K; ++a) x[a] = double(w[a]);
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        double t = 0.0;
+#pragma unroll
+        for (int a = 0; a < K; ++a)
+            if (kZPow<L>.m[a][j] != 0.0) t = fmad(kZPow<L>.m[a][j], x[a], t);
+        o[j] = T(t);
     }
 }
 
@@ -389,6 +420,26 @@ template <typename T, typename TA, int L>
 __device__ __forceinline__ void combine_out(const CombArgs<T, TA> &A, const SolveDev<T, TA> &S, int v,
                                             const TA (&acc)[(L + 1) * (L + 1)], const TA *__restrict__ wsum);
 
+// chunk-summed pair terms from the Z basis of k_prop back to the scaled
+// harmonics: x_a <- sum_j M_aj x_j
+template <typename TA, int L>
+__device__ __forceinline__ void zpow_to_harmonic(TA (&x)[(L + 1) * (L + 1)])
+{
+    constexpr int K = (L + 1) * (L + 1);
+    TA y[K];
+#pragma unroll
+    for (int a = 0; a < K; ++a) {
+        TA o = TA(0);
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+            if (kZPow<L>.m[a][j] != 0.0) o = fmad(TA(kZPow<L>.m[a][j]), x[j], o);
+        y[a] = o;
+    }
+#pragma unroll
+    for (int a = 0; a < K; ++a) x[a] = y[a];
+}
+
+
 template <typename T, typename TA, int L>
 __global__ __launch_bounds__(kWave) void k_combine(CombArgs<T, TA> A, const SolveDev<T, TA> *__restrict__ sv)
 {
@@ -408,6 +459,7 @@ __global__ __launch_bounds__(kWave) void k_combine(CombArgs<T, TA> A, const Solv
 #pragma unroll
         for (int a = 0; a < K; ++a) acc[a] += TA(p[a]);
     }
+    zpow_to_harmonic<TA, L>(acc);
 #pragma unroll
     for (int a = 0; a < K; ++a) acc[a] *= A.nsc[a];
     // self block: sum_k coef_k (M_k w), coef_k = (-mu)^{k-1}/k!
@@ -570,6 +622,7 @@ __global__ __launch_bounds__(kWave) void k_combine_c(CombArgs<T, TA> A, const So
             for (int a = 0; a < K; ++a) acc[a] += TA(p[a]);
         }
         // the steps of k_combine, in its order
+        zpow_to_harmonic<TA, L>(acc);
 #pragma unroll
         for (int a = 0; a < K; ++a) acc[a] *= A.nsc[a];
 #pragma unroll
@@ -976,6 +1029,15 @@ hipError_t launch_fill(T *p, size_t n, T v, hipStream_t st)
 }
 
 template <typename T, typename TA>
+hipError_t launch_to_zpow(int L, int N, const SolveDev<T, TA> *sv, int nsolve, hipStream_t st)
+{
+    if (nsolve < 1 || nsolve > 65535) return hipErrorInvalidValue;
+    const dim3 grid((N + 255) / 256, nsolve), block(256);
+    RTE_DISPATCH_L(L, hipLaunchKernelGGL((k_to_zpow<T, TA, LL>), grid, block, 0, st, N, sv));
+    return hipGetLastError();
+}
+
+template <typename T, typename TA>
 hipError_t launch_prop(int L, int path, int nr, const PropArgs<T> &A, const SolveDev<T, TA> *sv, int ngroups,
                        hipStream_t st)
 {
@@ -983,6 +1045,8 @@ hipError_t launch_prop(int L, int path, int nr, const PropArgs<T> &A, const Solv
     // several right-hand sides per group only on the mixed-precision cached path
     constexpr bool multi = sizeof(T) == 4 && sizeof(TA) == 8;
     if (nr < 1 || nr > kMaxNR || (nr > 1 && (!multi || path != 0))) return hipErrorInvalidValue;
+    hipError_t e = launch_to_zpow<T, TA>(L, A.g.N, sv, ngroups * nr, st);
+    if (e != hipSuccess) return e;
     RTE_DISPATCH_L(L, {
         if (path == 1) {
             hipLaunchKernelGGL((k_prop<T, TA, LL, 1, 1>), grid, block, 0, st, A, sv);
@@ -1008,6 +1072,8 @@ hipError_t launch_prop_sep(int L, int nr, const PropArgs<T> &A, const SolveDev<T
     dim3 grid((A.g.nblk + kPropWaves - 1) / kPropWaves, A.nchunk, ngroups), block(kWave * kPropWaves);
     if constexpr (sizeof(T) == 4 && sizeof(TA) == 8) {
         if (nr != 2 && nr != 3) return hipErrorInvalidValue;
+        hipError_t e = launch_to_zpow<T, TA>(L, A.g.N, sv, ngroups * nr, st);
+        if (e != hipSuccess) return e;
         RTE_DISPATCH_L(L, {
             if (nr == 2)
                 hipLaunchKernelGGL((k_prop<T, TA, LL, 0, 2, false, true>), grid, block, 0, st, A, sv);
@@ -1025,6 +1091,8 @@ hipError_t launch_prop_c(int L, int path, const PropArgs<T> &A, const SolveDev<T
                          hipStream_t st)
 {
     dim3 grid((A.g.nblk + kPropWaves - 1) / kPropWaves, A.nchunk, ncomplex), block(kWave * kPropWaves);
+    hipError_t e = launch_to_zpow<T, TA>(L, A.g.N, sv, 2 * ncomplex, st);
+    if (e != hipSuccess) return e;
     RTE_DISPATCH_L(L, {
         if (path == 1)
             hipLaunchKernelGGL((k_prop<T, TA, LL, 1, 2, true>), grid, block, 0, st, A, sv);
