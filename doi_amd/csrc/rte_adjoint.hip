@@ -20,13 +20,16 @@
 
 namespace rte {
 
+// Up to kMaxNC adjoint series (cotangents) share each propagation launch: the
+// pair geometry, its z-power values and the projection of the forward term are
+// formed once per pair for all of them.
 template <typename T, typename TA>
 struct AdjDev {
-    const T *X;      // [N][K] adjoint input, scaled and parity flipped: n_a (-1)^l xi_a
-    const T *Wq;     // [N][K] forward term W_q, scaled: n_a W_a
-    T *part;         // [nchunk][N][K]
-    T *pairbuf;      // [N][npad] pair weights w(d = source slot row, s = lane)
-    const T *ktab;   // [N][npad]
+    const T *Xz[kMaxNC];  // [N][K] adjoint inputs n_a (-1)^l xi_a, in the z-power basis
+    const T *Wqz;         // [N][K] forward term n_a W_a, in the z-power basis
+    T *part[kMaxNC];      // [nchunk][N][K] (z-power basis; k_adj_combine turns the sum back)
+    T *pairbuf[kMaxNC];   // [N][npad] pair weights w(d = source slot row, s = lane)
+    const T *ktab;        // [N][npad]
 };
 
 template <int L>
@@ -39,13 +42,17 @@ __device__ __forceinline__ int l_of(int a)
 }
 
 // ---------------------------------------------------------------------------
-// A_off applied to P xi (uniform input u, lane output o) plus the pair
-// weights pairbuf[u][slot(o)] += k (R.P xi(u)) (R.P W_q(o)).
+// A_off applied to P xi_c (uniform input u, lane output o) for NC cotangents c,
+// plus the pair weights pairbuf_c[u][slot(o)] += k (Z.P xi_c(u)) (Z.P W_q(o)).
+// In the z-power basis Z(u) of the forward kernel: P acts on it as on the
+// harmonics, (-1)^l on entry (l, m), since Z_l,m(-u) = (-1)^l Z_l,m(u), and the
+// pair weight, a product of two projections, does not depend on the basis.
 // grid = (nblk, nchunk), block = 64.
 // ---------------------------------------------------------------------------
-template <typename T, typename TA, int L>
+template <typename T, typename TA, int L, int NC>
 __global__ __launch_bounds__(kWave) void k_prop_adj(PropArgs<T> A, AdjDev<T, TA> D)
 {
+#pragma clang fp contract(off)
     constexpr int K = (L + 1) * (L + 1);
     const GridDev<T> &g = A.g;
     const int lane = threadIdx.x, blk = blockIdx.x, chunk = blockIdx.y;
@@ -59,34 +66,44 @@ __global__ __launch_bounds__(kWave) void k_prop_adj(PropArgs<T> A, AdjDev<T, TA>
     const int s1 = min(g.N, s0 + A.chunk_len);
     const int nxy = g.nx * g.ny;
 
-    T acc[K], wl[K];
+    T acc[NC][K], wl[K];
 #pragma unroll
     for (int a = 0; a < K; ++a) {
-        acc[a] = T(0);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc[c][a] = T(0);
         const T sgn = (l_of<L>(a) & 1) ? T(-1) : T(1);
-        wl[a] = sgn * D.Wq[(size_t)vo * K + a];
+        wl[a] = sgn * D.Wqz[(size_t)vo * K + a];
     }
-    const cptr<T> X = as_const<T>(D.X);
+    cptr<T> X[NC];
+    T *pb[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        X[c] = as_const<T>(D.Xz[c]);
+        pb[c] = D.pairbuf[c] + slot;
+    }
     const gptr<T> ktab = as_global<T>(D.ktab);
     const T dxc = T(cdi) * g.dx, dyc = T(cdj) * g.dy, dzc = T(cdk) * g.dz;
     int si = s0 % g.nx, sj = (s0 / g.nx) % g.ny, sk = s0 / nxy;
     for (int s = s0; s < s1; ++s) {
-        T w[K];
-#pragma unroll
-        for (int a = 0; a < K; ++a) w[a] = X[(size_t)s * K + a];
-        const T ddx = dxc - T(si) * g.dx, ddy = dyc - T(sj) * g.dy, ddz = dzc - T(sk) * g.dz;
-        T r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-        const bool self = (cdi == si) && (cdj == sj) && (cdk == sk);
+        const T ddx = fmad(-T(si), g.dx, dxc), ddy = fmad(-T(sj), g.dy, dyc), ddz = fmad(-T(sk), g.dz, dzc);
+        // the self pair's table entry is 0 and its direction (0, 0, 0): r^2 clamped as in k_prop
+        const T r2 = max(fmad(ddx, ddx, fmad(ddy, ddy, ddz * ddz)), T(1e-30));
         const T kf = ktab[(size_t)s * g.npad + slot];
-        r2 = self ? T(1) : r2;
         const T rinv = dev_rsqrt<T>(r2);
-        PairGeom<T, L> G;
-        pair_geom<T, L>(G, ddx * rinv, ddy * rinv, ddz * rinv);
-        const T t = kf * pair_dot<T, L>(w, G);
-        pair_axpy<T, L>(acc, t, G);
-        if (valid) {
-            T *pb = D.pairbuf + (size_t)s * g.npad + slot;
-            *pb += t * pair_dot<T, L>(wl, G);
+        PairZ<T, L> G;
+        pair_zpow<T, L>(G, ddx * rinv, ddy * rinv, ddz * rinv);
+        const T dl = pair_dot_z<T, L>(wl, G);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            T w[K];
+#pragma unroll
+            for (int a = 0; a < K; ++a) w[a] = X[c][(size_t)s * K + a];
+            const T t = kf * pair_dot_z<T, L>(w, G);
+            pair_axpy_z<T, L>(acc[c], t, G);
+            if (valid) {
+                T *q = pb[c] + (size_t)s * g.npad;
+                *q = fmad(t, dl, *q);
+            }
         }
         if (++si == g.nx) {
             si = 0;
@@ -97,10 +114,37 @@ __global__ __launch_bounds__(kWave) void k_prop_adj(PropArgs<T> A, AdjDev<T, TA>
         }
     }
     if (valid) {
-        T *__restrict__ out = D.part + ((size_t)chunk * g.N + vo) * K;
 #pragma unroll
-        for (int a = 0; a < K; ++a) out[a] = acc[a];
+        for (int c = 0; c < NC; ++c) {
+            T *__restrict__ out = D.part[c] + ((size_t)chunk * g.N + vo) * K;
+#pragma unroll
+            for (int a = 0; a < K; ++a) out[a] = acc[c][a];
+        }
     }
+}
+
+// [N][K] buffers into the z-power basis, out_i = M^T in_i (one thread per voxel
+// and buffer; grid = (ceil(N / 256), n), block = 256)
+template <typename T>
+struct ZBufs {
+    const T *in[kMaxNC + 1];
+    T *out[kMaxNC + 1];
+};
+
+template <typename T, int L>
+__global__ __launch_bounds__(256) void k_bufs_to_zpow(int N, ZBufs<T> B)
+{
+    constexpr int K = (L + 1) * (L + 1);
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    const T *__restrict__ w = B.in[blockIdx.y] + (size_t)v * K;
+    T *__restrict__ o = B.out[blockIdx.y] + (size_t)v * K;
+    double x[K], y[K];
+#pragma unroll
+    for (int a = 0; a < K; ++a) x[a] = double(w[a]);
+    zpow_from_harmonic<double, L>(x, y);
+#pragma unroll
+    for (int j = 0; j < K; ++j) o[j] = T(y[j]);
 }
 
 // ---------------------------------------------------------------------------
@@ -128,6 +172,7 @@ __global__ __launch_bounds__(kWave) void k_adj_combine(AdjCombArgs<T, TA> A)
 #pragma unroll
         for (int a = 0; a < K; ++a) acc[a] += TA(p[a]);
     }
+    zpow_to_harmonic<TA, L>(acc);
 #pragma unroll
     for (int a = 0; a < K; ++a) {
         const TA sgn = (l_of<L>(a) & 1) ? TA(-1) : TA(1);
@@ -453,12 +498,34 @@ __global__ void k_accumulate(size_t n, const TA *__restrict__ x, TA *__restrict_
     }
 
 template <typename T, typename TA>
-hipError_t launch_adj_prop(int L, const PropArgs<T> &A, const T *X, const T *Wq, T *part, T *pairbuf,
-                           const T *ktab, hipStream_t st)
+hipError_t launch_adj_prop(int L, const PropArgs<T> &A, int nc, const T *const *X, T *const *Xz, const T *Wq, T *Wqz,
+                           T *const *part, T *const *pairbuf, const T *ktab, hipStream_t st)
 {
-    AdjDev<T, TA> D{X, Wq, part, pairbuf, ktab};
+    if (nc < 1 || nc > kMaxNC) return hipErrorInvalidValue;
+    // the inputs into the z-power basis: the forward term and each cotangent's xi
+    ZBufs<T> B{};
+    B.in[0] = Wq;
+    B.out[0] = Wqz;
+    AdjDev<T, TA> D{};
+    D.Wqz = Wqz;
+    D.ktab = ktab;
+    for (int c = 0; c < nc; ++c) {
+        B.in[1 + c] = X[c];
+        B.out[1 + c] = Xz[c];
+        D.Xz[c] = Xz[c];
+        D.part[c] = part[c];
+        D.pairbuf[c] = pairbuf[c];
+    }
+    RTE_DISPATCH_L2(L, hipLaunchKernelGGL((k_bufs_to_zpow<T, LL>), dim3((A.g.N + 255) / 256, 1 + nc), dim3(256), 0,
+                                          st, A.g.N, B));
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
     dim3 grid(A.g.nblk, A.nchunk), block(kWave);
-    RTE_DISPATCH_L2(L, hipLaunchKernelGGL((k_prop_adj<T, TA, LL>), grid, block, 0, st, A, D));
+    if (nc == 1) {
+        RTE_DISPATCH_L2(L, hipLaunchKernelGGL((k_prop_adj<T, TA, LL, 1>), grid, block, 0, st, A, D));
+    } else {
+        RTE_DISPATCH_L2(L, hipLaunchKernelGGL((k_prop_adj<T, TA, LL, 2>), grid, block, 0, st, A, D));
+    }
     return hipGetLastError();
 }
 
@@ -559,10 +626,12 @@ template hipError_t launch_ref_source<double, double>(int, const GridDev<double>
                                                       const int *, int, double, double, const double *, const double *,
                                                       const SolveDev<double, double> *, int, hipStream_t);
 template hipError_t launch_accumulate<double>(size_t, const double *, double *, hipStream_t);
-template hipError_t launch_adj_prop<float, double>(int, const PropArgs<float> &, const float *, const float *,
-                                                   float *, float *, const float *, hipStream_t);
-template hipError_t launch_adj_prop<double, double>(int, const PropArgs<double> &, const double *, const double *,
-                                                    double *, double *, const double *, hipStream_t);
+template hipError_t launch_adj_prop<float, double>(int, const PropArgs<float> &, int, const float *const *,
+                                                   float *const *, const float *, float *, float *const *,
+                                                   float *const *, const float *, hipStream_t);
+template hipError_t launch_adj_prop<double, double>(int, const PropArgs<double> &, int, const double *const *,
+                                                    double *const *, const double *, double *, double *const *,
+                                                    double *const *, const double *, hipStream_t);
 template hipError_t launch_adj_combine<float, double>(int, const AdjCombArgs<float, double> &, hipStream_t);
 template hipError_t launch_adj_combine<double, double>(int, const AdjCombArgs<double, double> &, hipStream_t);
 template hipError_t launch_adj_init<float, double>(int, const GridDev<float> &, const double *, const Scales &,

@@ -98,87 +98,9 @@ __device__ __forceinline__ T march_tau(P mut, int v, int ix, int iy, int iz, int
 }
 
 // ---------------------------------------------------------------------------
-// One (destination, source) pair in the factored real-SH form, on scaled
-// coefficients (see sh_gamma):  p = sum_a Rt_a(u) w'_a ;  acc' += (kf p) Rt(u)
-// with Rt_l0 = Pt_l^0, Rt_lm = Pt_l^m C_m, Rt_l,-m = Pt_l^m S_m,
-// C_m + i S_m = (x + i y)^m.  The adjoint propagation (k_prop_adj) uses this
-// form; the forward propagation uses the expanded one below.
-// ---------------------------------------------------------------------------
-template <typename T, int L>
-struct PairGeom {
-    T C[L + 1], S[L + 1];
-    T P[L + 1][L + 1];  // only l >= m used
-};
-
-template <typename T, int L>
-__device__ __forceinline__ void pair_geom(PairGeom<T, L> &G, T x, T y, T z)
-{
-    G.C[0] = T(1);
-    G.S[0] = T(0);
-    if constexpr (L >= 1) {
-        G.C[1] = x;
-        G.S[1] = y;
-    }
-#pragma unroll
-    for (int m = 2; m <= L; ++m) {
-        G.C[m] = G.C[m - 1] * x - G.S[m - 1] * y;
-        G.S[m] = G.S[m - 1] * x + G.C[m - 1] * y;
-    }
-#pragma unroll
-    for (int m = 0; m <= L; ++m) {
-        G.P[m][m] = T(1);
-        if (m + 1 <= L) G.P[m + 1][m] = z;
-#pragma unroll
-        for (int l = m + 2; l <= L; ++l) G.P[l][m] = z * G.P[l - 1][m] - sh_gamma<T>(l, m) * G.P[l - 2][m];
-    }
-}
-
-// p = sum_a Rt_a(u) w'_a
-template <typename T, int L>
-__device__ __forceinline__ T pair_dot(const T (&w)[(L + 1) * (L + 1)], const PairGeom<T, L> &G)
-{
-    // two partial sums: halves the dependent chain of the dot product
-    T p = w[0], q = T(0);
-#pragma unroll
-    for (int l = 1; l <= L; ++l) p += G.P[l][0] * w[l * l + l];
-#pragma unroll
-    for (int m = 1; m <= L; ++m) {
-        T A = w[m * m + m + m], B = w[m * m + m - m];
-#pragma unroll
-        for (int l = m + 1; l <= L; ++l) {
-            A += G.P[l][m] * w[l * l + l + m];
-            B += G.P[l][m] * w[l * l + l - m];
-        }
-        q += G.C[m] * A;
-        q += G.S[m] * B;
-    }
-    return p + q;
-}
-
-// acc' += t Rt(u)
-template <typename T, int L>
-__device__ __forceinline__ void pair_axpy(T (&acc)[(L + 1) * (L + 1)], T t, const PairGeom<T, L> &G)
-{
-    acc[0] += t;
-#pragma unroll
-    for (int l = 1; l <= L; ++l) acc[l * l + l] += t * G.P[l][0];
-#pragma unroll
-    for (int m = 1; m <= L; ++m) {
-        const T u = t * G.C[m], vv = t * G.S[m];
-        acc[m * m + m + m] += u;
-        acc[m * m + m - m] += vv;
-#pragma unroll
-        for (int l = m + 1; l <= L; ++l) {
-            acc[l * l + l + m] += u * G.P[l][m];
-            acc[l * l + l - m] += vv * G.P[l][m];
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// The same pair for the propagation kernel (every group, real or complex), in
-// the basis of the scaled real harmonics with their Legendre factor replaced by
-// a power of z:
+// One (destination, source) pair for the propagation kernels (forward, every
+// group, real or complex, and adjoint), in the basis of the scaled real
+// harmonics with their Legendre factor replaced by a power of z:
 //   Z_l,m = C_m z^(l-m),  Z_l,-m = S_m z^(l-m)  (C_m + i S_m = (x + i y)^m),
 // stored where Rt_l,+-m = C_m Pt_l^m, S_m Pt_l^m sit (a = l^2 + l +- m).  For
 // each m the powers z^0 .. z^(L-m) span the same polynomials as the monic
@@ -222,6 +144,80 @@ struct ZPowBasis {
 
 template <int L>
 inline constexpr ZPowBasis<L> kZPow{};
+
+// M's nonzero entries as compile-time lists, in the two orders the conversions
+// accumulate in: by z-power index j, then harmonic index a (for M^T x), and by a,
+// then j (for M x).  A fold over the list expands every term with its constant,
+// whatever the optimiser's unrolling limits.
+template <int L>
+struct ZPowTerms {
+    static constexpr int K = (L + 1) * (L + 1);
+    static constexpr int count()
+    {
+        int c = 0;
+        for (int a = 0; a < K; ++a)
+            for (int j = 0; j < K; ++j) c += kZPow<L>.m[a][j] != 0.0;
+        return c;
+    }
+    static constexpr int NZ = count();
+    int a[NZ], j[NZ];
+    double v[NZ];
+    constexpr ZPowTerms(bool by_j) : a{}, j{}, v{}
+    {
+        int c = 0;
+        for (int p = 0; p < K; ++p)
+            for (int q = 0; q < K; ++q) {
+                const int aa = by_j ? q : p, jj = by_j ? p : q;
+                if (kZPow<L>.m[aa][jj] != 0.0) {
+                    a[c] = aa;
+                    j[c] = jj;
+                    v[c] = kZPow<L>.m[aa][jj];
+                    ++c;
+                }
+            }
+    }
+};
+
+template <int L>
+inline constexpr ZPowTerms<L> kZPowByJ{true};
+template <int L>
+inline constexpr ZPowTerms<L> kZPowByA{false};
+
+template <typename TA, int L, int... I>
+__device__ __forceinline__ void zpow_from_terms(const TA *x, TA *y, std::integer_sequence<int, I...>)
+{
+    ((y[kZPowByJ<L>.j[I]] = fmad(TA(kZPowByJ<L>.v[I]), x[kZPowByJ<L>.a[I]], y[kZPowByJ<L>.j[I]])), ...);
+}
+
+template <typename TA, int L, int... I>
+__device__ __forceinline__ void zpow_to_terms(const TA *x, TA *y, std::integer_sequence<int, I...>)
+{
+    ((y[kZPowByA<L>.a[I]] = fmad(TA(kZPowByA<L>.v[I]), x[kZPowByA<L>.j[I]], y[kZPowByA<L>.a[I]])), ...);
+}
+
+// harmonic-basis coefficients into the z-power basis, y_j = sum_a M_aj x_a
+// (a ascending for each j)
+template <typename TA, int L>
+__device__ __forceinline__ void zpow_from_harmonic(const TA (&x)[(L + 1) * (L + 1)], TA (&y)[(L + 1) * (L + 1)])
+{
+#pragma unroll
+    for (int q = 0; q < (L + 1) * (L + 1); ++q) y[q] = TA(0);
+    zpow_from_terms<TA, L>(x, y, std::make_integer_sequence<int, ZPowTerms<L>::NZ>{});
+}
+
+// sums of z-power terms back to the harmonic basis: x_a <- sum_j M_aj x_j
+// (j ascending for each a)
+template <typename TA, int L>
+__device__ __forceinline__ void zpow_to_harmonic(TA (&x)[(L + 1) * (L + 1)])
+{
+    constexpr int K = (L + 1) * (L + 1);
+    TA y[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) y[q] = TA(0);
+    zpow_to_terms<TA, L>(x, y, std::make_integer_sequence<int, ZPowTerms<L>::NZ>{});
+#pragma unroll
+    for (int q = 0; q < K; ++q) x[q] = y[q];
+}
 
 template <typename T, int L>
 struct PairZ {

@@ -1435,7 +1435,12 @@ struct AdjointRun {
     HostSolve h{};
     SolveDev<T, TA> sv{};
     std::vector<TA> E;
-    DevBuf dW, dWp, dS0, dPart, dPix, dRed, dSv, dBeam, dCot, dCoefE, dX, dWq, dPair, dGt, dGs, dGa2, dGs2, dLam0;
+    DevBuf dW, dWp, dS0, dPart, dPix, dRed, dSv, dBeam, dCoefE, dWq, dWqz;
+    // per cotangent of a backward pass (up to kMaxNC share its propagation launches;
+    // the first one's chunk partials reuse the forward's dPart)
+    DevBuf dCot[kMaxNC], dX[kMaxNC], dXz[kMaxNC], dPartC[kMaxNC], dPair[kMaxNC], dGt[kMaxNC], dGs[kMaxNC],
+        dGa2[kMaxNC], dGs2[kMaxNC], dLam0[kMaxNC];
+    int nc_ready = 0;
     std::vector<std::unique_ptr<DevBuf>> ckpt, seg;
     bool backward_ready = false;
 
@@ -1563,67 +1568,86 @@ struct AdjointRun {
         return RTE_OK;
     }
 
-    int prepare_backward()
+    int prepare_backward(int nc)
     {
-        if (backward_ready) return RTE_OK;
-        HIP_TRY(dCot.alloc(sizeof(TA) * P));
-        HIP_TRY(dCoefE.alloc(sizeof(TA) * (kMaxL + 1)));
-        TA cE[kMaxL + 1];
-        for (int l = 0; l <= kMaxL; ++l) cE[l] = l <= L ? TA(coefE[l]) : TA(0);
-        HIP_TRY(hipMemcpy(dCoefE.p, cE, sizeof cE, hipMemcpyHostToDevice));
-        HIP_TRY(dX.alloc(F * sizeof(T)));
-        HIP_TRY(dWq.alloc(F * sizeof(T)));
-        HIP_TRY(dPair.alloc((size_t)N * g.npad * sizeof(T)));
-        HIP_TRY(dGt.alloc(N * sizeof(TA)));
-        HIP_TRY(dGs.alloc(N * sizeof(TA)));
-        HIP_TRY(dGa2.alloc(N * sizeof(TA)));
-        HIP_TRY(dGs2.alloc(N * sizeof(TA)));
-        HIP_TRY(dLam0.alloc((size_t)N * (L + 1) * sizeof(TA)));
-        for (int i = 0; i + 1 < stride; ++i) {
-            seg.emplace_back(new DevBuf());
-            HIP_TRY(seg.back()->alloc(F * sizeof(TA)));
+        if (!backward_ready) {
+            HIP_TRY(dCoefE.alloc(sizeof(TA) * (kMaxL + 1)));
+            TA cE[kMaxL + 1];
+            for (int l = 0; l <= kMaxL; ++l) cE[l] = l <= L ? TA(coefE[l]) : TA(0);
+            HIP_TRY(hipMemcpy(dCoefE.p, cE, sizeof cE, hipMemcpyHostToDevice));
+            HIP_TRY(dWq.alloc(F * sizeof(T)));
+            HIP_TRY(dWqz.alloc(F * sizeof(T)));
+            for (int i = 0; i + 1 < stride; ++i) {
+                seg.emplace_back(new DevBuf());
+                HIP_TRY(seg.back()->alloc(F * sizeof(TA)));
+            }
+            backward_ready = true;
         }
-        backward_ready = true;
+        for (int c = nc_ready; c < nc; ++c) {
+            HIP_TRY(dCot[c].alloc(sizeof(TA) * P));
+            HIP_TRY(dX[c].alloc(F * sizeof(T)));
+            HIP_TRY(dXz[c].alloc(F * sizeof(T)));
+            if (c > 0) HIP_TRY(dPartC[c].alloc((size_t)nch * F * sizeof(T)));
+            HIP_TRY(dPair[c].alloc((size_t)N * g.npad * sizeof(T)));
+            HIP_TRY(dGt[c].alloc(N * sizeof(TA)));
+            HIP_TRY(dGs[c].alloc(N * sizeof(TA)));
+            HIP_TRY(dGa2[c].alloc(N * sizeof(TA)));
+            HIP_TRY(dGs2[c].alloc(N * sizeof(TA)));
+            HIP_TRY(dLam0[c].alloc((size_t)N * (L + 1) * sizeof(TA)));
+            nc_ready = c + 1;
+        }
         return RTE_OK;
     }
 
-    // d(cot . E)/d mu_a, d mu_s for every voxel
-    int backward(const double *cot_host, double *dmu_a, double *dmu_s)
+    // d(cot . E)/d mu_a, d mu_s for every voxel, for nc <= kMaxNC cotangents at once:
+    // their adjoint series share every propagation launch
+    int backward_multi(int nc, const double *const *cot_host, double *const *dmu_a, double *const *dmu_s)
     {
+        if (nc < 1 || nc > kMaxNC) return fail(RTE_E_ARG, "backward: %d cotangents per pass (1..%d)", nc, kMaxNC);
         int rc;
-        if ((rc = prepare_backward())) return rc;
+        if ((rc = prepare_backward(nc))) return rc;
         hipStream_t st = ctx->stream;
         const rte_grid &grid = m->grid;
-        std::vector<TA> cot(P);
-        for (int i = 0; i < P; ++i) cot[i] = TA(cot_host[i]);
-        HIP_TRY(hipMemcpyAsync(dCot.p, cot.data(), sizeof(TA) * P, hipMemcpyHostToDevice, st));
-        HIP_TRY(launch_fill<T>(dPair.as<T>(), (size_t)N * g.npad, T(0), st));
-        HIP_TRY(launch_fill<TA>(dGt.as<TA>(), N, TA(0), st));
-        HIP_TRY(launch_fill<TA>(dGs.as<TA>(), N, TA(0), st));
-        HIP_TRY(launch_fill<TA>(dGa2.as<TA>(), N, TA(0), st));
-        HIP_TRY(launch_fill<TA>(dGs2.as<TA>(), N, TA(0), st));
-        HIP_TRY(launch_adj_init<T, TA>(L, g, dCot.as<TA>(), sc, dCoefE.as<TA>(), dX.as<T>(), st));
         // the anisotropy exactly as the forward scatter used it (compute precision)
         const TA gan = TA(T(m->g));
-        AdjCombArgs<T, TA> aa;
-        std::memset(&aa, 0, sizeof aa);
-        aa.g = g;
-        aa.nchunk = nch;
-        aa.mom = ca.mom;
-        aa.momT = ca.momT;
-        aa.self_scale = ca.self_scale;
-        aa.dV = ca.dV;
-        for (int l = 0; l <= kMaxL; ++l) aa.coefE[l] = ca.coefE[l];
-        for (int a = 0; a < kMaxK; ++a) aa.nsc[a] = ca.nsc[a];
-        aa.X = dX.as<T>();
-        aa.Wq = dWq.as<T>();
-        aa.part = dPart.as<T>();
-        aa.mut = sv.mut;
-        aa.mus = sv.mus;
-        aa.g_aniso = gan;
-        aa.cot = dCot.as<TA>();
-        aa.gt = dGt.as<TA>();
-        aa.gs = dGs.as<TA>();
+        AdjCombArgs<T, TA> aa[kMaxNC];
+        const T *Xc[kMaxNC];
+        T *Xzc[kMaxNC], *partc[kMaxNC], *pairc[kMaxNC];
+        for (int c = 0; c < nc; ++c) {
+            std::vector<TA> cot(P);
+            for (int i = 0; i < P; ++i) cot[i] = TA(cot_host[c][i]);
+            HIP_TRY(hipMemcpyAsync(dCot[c].p, cot.data(), sizeof(TA) * P, hipMemcpyHostToDevice, st));
+            HIP_TRY(hipStreamSynchronize(st));  // cot lives on this frame
+            HIP_TRY(launch_fill<T>(dPair[c].as<T>(), (size_t)N * g.npad, T(0), st));
+            HIP_TRY(launch_fill<TA>(dGt[c].as<TA>(), N, TA(0), st));
+            HIP_TRY(launch_fill<TA>(dGs[c].as<TA>(), N, TA(0), st));
+            HIP_TRY(launch_fill<TA>(dGa2[c].as<TA>(), N, TA(0), st));
+            HIP_TRY(launch_fill<TA>(dGs2[c].as<TA>(), N, TA(0), st));
+            HIP_TRY(launch_adj_init<T, TA>(L, g, dCot[c].as<TA>(), sc, dCoefE.as<TA>(), dX[c].as<T>(), st));
+            AdjCombArgs<T, TA> &A = aa[c];
+            std::memset(&A, 0, sizeof A);
+            A.g = g;
+            A.nchunk = nch;
+            A.mom = ca.mom;
+            A.momT = ca.momT;
+            A.self_scale = ca.self_scale;
+            A.dV = ca.dV;
+            for (int l = 0; l <= kMaxL; ++l) A.coefE[l] = ca.coefE[l];
+            for (int a = 0; a < kMaxK; ++a) A.nsc[a] = ca.nsc[a];
+            A.X = dX[c].as<T>();
+            A.Wq = dWq.as<T>();
+            A.part = c == 0 ? dPart.as<T>() : dPartC[c].as<T>();
+            A.mut = sv.mut;
+            A.mus = sv.mus;
+            A.g_aniso = gan;
+            A.cot = dCot[c].as<TA>();
+            A.gt = dGt[c].as<TA>();
+            A.gs = dGs[c].as<TA>();
+            Xc[c] = dX[c].as<T>();
+            Xzc[c] = dXz[c].as<T>();
+            partc[c] = const_cast<T *>(A.part);
+            pairc[c] = dPair[c].as<T>();
+        }
         SolveDev<T, TA> *svd = dSv.as<SolveDev<T, TA>>();
         const int nseg = (n + stride - 1) / stride;
         for (int j = nseg - 1; j >= 0; --j) {
@@ -1654,27 +1678,36 @@ struct AdjointRun {
                     HIP_TRY(hipMemcpyAsync(dWq.p, dS0.p, F * sizeof(T), hipMemcpyDeviceToDevice, st));
                 else
                     HIP_TRY(launch_make_wq<T, TA>(L, g, sc, gan, sv.mus, awp, dWq.as<T>(), st));
-                HIP_TRY(launch_adj_prop<T, TA>(L, pa, dX.as<T>(), dWq.as<T>(), dPart.as<T>(), dPair.as<T>(), sv.ktab,
+                HIP_TRY(launch_adj_prop<T, TA>(L, pa, nc, Xc, Xzc, dWq.as<T>(), dWqz.as<T>(), partc, pairc, sv.ktab,
                                                st));
-                aa.aw_prev = awp;
-                aa.lam0 = (qq == 0) ? dLam0.as<TA>() : nullptr;
-                HIP_TRY(launch_adj_combine<T, TA>(L, aa, st));
+                for (int c = 0; c < nc; ++c) {
+                    aa[c].aw_prev = awp;
+                    aa[c].lam0 = (qq == 0) ? dLam0[c].as<TA>() : nullptr;
+                    HIP_TRY(launch_adj_combine<T, TA>(L, aa[c], st));
+                }
             }
         }
-        HIP_TRY(launch_backproject<T, TA>(g, nch, clen, pa.thr2, dPair.as<T>(), TA(-dV), dGt.as<TA>(), st));
-        HIP_TRY(launch_src_grad<TA>(L, grid.nx, grid.ny, grid.nz, dx, dy, dz, grid.xmin, grid.ymin, m->g, bd,
-                                    dLam0.as<TA>(), dCot.as<TA>(), dGa2.as<TA>(), dGs2.as<TA>(), st));
         std::vector<TA> gt(N), gs(N), ga2(N), gs2(N);
-        HIP_TRY(hipMemcpyAsync(gt.data(), dGt.p, sizeof(TA) * N, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipMemcpyAsync(gs.data(), dGs.p, sizeof(TA) * N, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipMemcpyAsync(ga2.data(), dGa2.p, sizeof(TA) * N, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipMemcpyAsync(gs2.data(), dGs2.p, sizeof(TA) * N, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        for (int v = 0; v < N; ++v) {
-            if (dmu_a) dmu_a[v] = (double)gt[v] + (double)ga2[v];
-            if (dmu_s) dmu_s[v] = (double)gt[v] + (double)gs[v] + (double)gs2[v];
+        for (int c = 0; c < nc; ++c) {
+            HIP_TRY(launch_backproject<T, TA>(g, nch, clen, pa.thr2, dPair[c].as<T>(), TA(-dV), dGt[c].as<TA>(), st));
+            HIP_TRY(launch_src_grad<TA>(L, grid.nx, grid.ny, grid.nz, dx, dy, dz, grid.xmin, grid.ymin, m->g, bd,
+                                        dLam0[c].as<TA>(), dCot[c].as<TA>(), dGa2[c].as<TA>(), dGs2[c].as<TA>(), st));
+            HIP_TRY(hipMemcpyAsync(gt.data(), dGt[c].p, sizeof(TA) * N, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipMemcpyAsync(gs.data(), dGs[c].p, sizeof(TA) * N, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipMemcpyAsync(ga2.data(), dGa2[c].p, sizeof(TA) * N, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipMemcpyAsync(gs2.data(), dGs2[c].p, sizeof(TA) * N, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            for (int v = 0; v < N; ++v) {
+                if (dmu_a && dmu_a[c]) dmu_a[c][v] = (double)gt[v] + (double)ga2[v];
+                if (dmu_s && dmu_s[c]) dmu_s[c][v] = (double)gt[v] + (double)gs[v] + (double)gs2[v];
+            }
         }
         return RTE_OK;
+    }
+
+    int backward(const double *cot_host, double *dmu_a, double *dmu_s)
+    {
+        return backward_multi(1, &cot_host, &dmu_a, &dmu_s);
     }
 
     void fill_stats(rte_stats *stats)
@@ -1832,19 +1865,35 @@ static int fisher_run(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const r
     if (method == 1) {
         if ((rc = ref_jacobian_run<T, TA>(ctx, m, beam, p, part, nparam, wrt, J, E, stats))) return rc;
     } else {
-        // exact reverse mode: one adjoint series per detector
+        // exact reverse mode: one adjoint series per detector, kMaxNC detectors per
+        // backward pass (their series share the propagation launches)
         AdjointRun<T, TA> run;
         if ((rc = run.forward(ctx, m, beam, p, stride))) return rc;
-        std::vector<double> cot(P, 0.0), ga(N), gs(N);
+        std::vector<int> pixels;
         for (int pix = 0; pix < P; ++pix) {
             E[pix] = (double)run.E[pix];
-            if (!(E[pix] != 0.0)) continue;
-            cot.assign(P, 0.0);
-            cot[pix] = 1.0;
-            if ((rc = run.backward(cot.data(), ga.data(), gs.data()))) return rc;
-            const std::vector<double> &gv = wrt == 0 ? ga : gs;
-            for (int v = 0; v < N; ++v)
-                if (part[v] >= 0) J[(size_t)pix * nparam + part[v]] += gv[v];
+            if (E[pix] != 0.0) pixels.push_back(pix);
+        }
+        std::vector<std::vector<double>> cot(kMaxNC, std::vector<double>(P, 0.0)), ga(kMaxNC, std::vector<double>(N)),
+            gs(kMaxNC, std::vector<double>(N));
+        for (size_t o = 0; o < pixels.size(); o += kMaxNC) {
+            const int nc = (int)std::min<size_t>(kMaxNC, pixels.size() - o);
+            const double *cp[kMaxNC];
+            double *ap[kMaxNC], *sp[kMaxNC];
+            for (int c = 0; c < nc; ++c) {
+                cot[c].assign(P, 0.0);
+                cot[c][pixels[o + c]] = 1.0;
+                cp[c] = cot[c].data();
+                ap[c] = ga[c].data();
+                sp[c] = gs[c].data();
+            }
+            if ((rc = run.backward_multi(nc, cp, ap, sp))) return rc;
+            for (int c = 0; c < nc; ++c) {
+                const std::vector<double> &gv = wrt == 0 ? ga[c] : gs[c];
+                const int pix = pixels[o + c];
+                for (int v = 0; v < N; ++v)
+                    if (part[v] >= 0) J[(size_t)pix * nparam + part[v]] += gv[v];
+            }
         }
         run.fill_stats(stats);
     }

@@ -193,9 +193,11 @@ struct AdjCombArgs {
     TA *gs;               // [N] d/d mu_s accumulator (scatter)
     TA *lam0;             // [N][L+1] Lambda_0 m = 0 coefficients when q == 0, else nullptr
 };
+// adjoint series (cotangents) propagated by one launch
+constexpr int kMaxNC = 2;
 template <typename T, typename TA>
-hipError_t launch_adj_prop(int L, const PropArgs<T> &A, const T *X, const T *Wq, T *part, T *pairbuf,
-                           const T *ktab, hipStream_t st);
+hipError_t launch_adj_prop(int L, const PropArgs<T> &A, int nc, const T *const *X, T *const *Xz, const T *Wq, T *Wqz,
+                           T *const *part, T *const *pairbuf, const T *ktab, hipStream_t st);
 template <typename T, typename TA>
 hipError_t launch_adj_combine(int L, const AdjCombArgs<T, TA> &A, hipStream_t st);
 template <typename T, typename TA>

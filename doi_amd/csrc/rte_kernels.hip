@@ -315,17 +315,12 @@ __global__ __launch_bounds__(256) void k_to_zpow(int N, const SolveDev<T, TA> *_
     const SolveDev<T, TA> &s = sv[blockIdx.y];
     const T *__restrict__ w = s.W + (size_t)v * K;
     T *__restrict__ o = s.Wp + (size_t)v * K;
-    double x[K];
+    double x[K], y[K];
 #pragma unroll
     for (int a = 0; a < K; ++a) x[a] = double(w[a]);
+    zpow_from_harmonic<double, L>(x, y);
 #pragma unroll
-    for (int j = 0; j < K; ++j) {
-        double t = 0.0;
-#pragma unroll
-        for (int a = 0; a < K; ++a)
-            if (kZPow<L>.m[a][j] != 0.0) t = fmad(kZPow<L>.m[a][j], x[a], t);
-        o[j] = T(t);
-    }
+    for (int j = 0; j < K; ++j) o[j] = T(y[j]);
 }
 
 // ---------------------------------------------------------------------------
@@ -419,25 +414,6 @@ __global__ __launch_bounds__(1024) void k_ktab(GridDev<T> g, int ntiles, T thr2,
 template <typename T, typename TA, int L>
 __device__ __forceinline__ void combine_out(const CombArgs<T, TA> &A, const SolveDev<T, TA> &S, int v,
                                             const TA (&acc)[(L + 1) * (L + 1)], const TA *__restrict__ wsum);
-
-// chunk-summed pair terms from the Z basis of k_prop back to the scaled
-// harmonics: x_a <- sum_j M_aj x_j
-template <typename TA, int L>
-__device__ __forceinline__ void zpow_to_harmonic(TA (&x)[(L + 1) * (L + 1)])
-{
-    constexpr int K = (L + 1) * (L + 1);
-    TA y[K];
-#pragma unroll
-    for (int a = 0; a < K; ++a) {
-        TA o = TA(0);
-#pragma unroll
-        for (int j = 0; j < K; ++j)
-            if (kZPow<L>.m[a][j] != 0.0) o = fmad(TA(kZPow<L>.m[a][j]), x[j], o);
-        y[a] = o;
-    }
-#pragma unroll
-    for (int a = 0; a < K; ++a) x[a] = y[a];
-}
 
 
 template <typename T, typename TA, int L>
