@@ -84,11 +84,25 @@ __global__ __launch_bounds__(kWave) void k_prop_adj(PropArgs<T> A, AdjDev<T, TA>
     const gptr<T> ktab = as_global<T>(D.ktab);
     const T dxc = T(cdi) * g.dx, dyc = T(cdj) * g.dy, dzc = T(cdk) * g.dz;
     int si = s0 % g.nx, sj = (s0 / g.nx) % g.ny, sk = s0 / nxy;
+    // the table entry and the pair weights of the next source are loaded one source
+    // ahead (the last source's again at the chunk's end: in bounds, unused); a lane
+    // outside the grid keeps its weights in registers and never stores them
+    T kf_next = s0 < s1 ? ktab[(size_t)s0 * g.npad + slot] : T(0), pw_next[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) pw_next[c] = s0 < s1 ? pb[c][(size_t)s0 * g.npad] : T(0);
     for (int s = s0; s < s1; ++s) {
         const T ddx = fmad(-T(si), g.dx, dxc), ddy = fmad(-T(sj), g.dy, dyc), ddz = fmad(-T(sk), g.dz, dzc);
         // the self pair's table entry is 0 and its direction (0, 0, 0): r^2 clamped as in k_prop
         const T r2 = max(fmad(ddx, ddx, fmad(ddy, ddy, ddz * ddz)), T(1e-30));
-        const T kf = ktab[(size_t)s * g.npad + slot];
+        const T kf = kf_next;
+        T pw[NC];
+        const int sn = (s + 1 < s1) ? s + 1 : s;
+        kf_next = ktab[(size_t)sn * g.npad + slot];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            pw[c] = pw_next[c];
+            pw_next[c] = pb[c][(size_t)sn * g.npad];
+        }
         const T rinv = dev_rsqrt<T>(r2);
         PairZ<T, L> G;
         pair_zpow<T, L>(G, ddx * rinv, ddy * rinv, ddz * rinv);
@@ -100,10 +114,7 @@ __global__ __launch_bounds__(kWave) void k_prop_adj(PropArgs<T> A, AdjDev<T, TA>
             for (int a = 0; a < K; ++a) w[a] = X[c][(size_t)s * K + a];
             const T t = kf * pair_dot_z<T, L>(w, G);
             pair_axpy_z<T, L>(acc[c], t, G);
-            if (valid) {
-                T *q = pb[c] + (size_t)s * g.npad;
-                *q = fmad(t, dl, *q);
-            }
+            if (valid) pb[c][(size_t)s * g.npad] = fmad(t, dl, pw[c]);
         }
         if (++si == g.nx) {
             si = 0;
