@@ -386,6 +386,86 @@ __global__ __launch_bounds__(1024) void k_backproject_sym(GridDev<T> g, int ntil
 }
 
 // ---------------------------------------------------------------------------
+// Back-projection onto parameter blocks (the exact Fisher information needs only
+// sums of the mu_t gradient over each block, part[v] in [0, nparam) or -1): for
+// NC backward passes at once, out_c[p] += scale * sum_pairs w_c(pair) len_p(pair),
+// len_p the length of the pair's ray inside block p.  The tiling and the march are
+// those of k_backproject_sym (unordered pairs, w_c = both directions' weights), one
+// march for all NC passes; a lane adds the lengths of consecutive voxels of one
+// block in a register and adds w_c times that run to the block's LDS accumulator
+// when the ray leaves the block.  The part map is read from LDS.
+// grid = ceil(ntiles / 16), block = 1024; LDS = N ints + NC nparam accumulators.
+// ---------------------------------------------------------------------------
+template <typename T, typename TA, int NC>
+__global__ __launch_bounds__(1024) void k_backproject_blocks(GridDev<T> g, int ntiles, T thr2, BlockBp<T, TA> B)
+{
+    extern __shared__ unsigned char smem_raw[];
+    int *spart = reinterpret_cast<int *>(smem_raw);
+    TA *sacc = reinterpret_cast<TA *>(smem_raw + (((size_t)g.N * sizeof(int) + 15) & ~size_t(15)));
+    const int tid = threadIdx.x, lane = tid & (kWave - 1);
+    for (int i = tid; i < g.N; i += blockDim.x) spart[i] = B.part[i];
+    for (int i = tid; i < NC * B.nparam; i += blockDim.x) sacc[i] = TA(0);
+    __syncthreads();
+    const int tile = blockIdx.x * (blockDim.x / kWave) + tid / kWave;
+    if (tile < ntiles) {
+        int bd = (int)((sqrtf(8.0f * (float)tile + 1.0f) - 1.0f) * 0.5f);
+        while (bd > 0 && bd * (bd + 1) / 2 > tile) --bd;
+        while ((bd + 1) * (bd + 2) / 2 <= tile) ++bd;
+        const int bs = tile - bd * (bd + 1) / 2;
+        int di, dj, dk;
+        lane_dest(g, bd, lane, di, dj, dk);
+        const bool dvalid = di < g.nx && dj < g.ny && dk < g.nz;
+        const int cdi = min(di, g.nx - 1), cdj = min(dj, g.ny - 1), cdk = min(dk, g.nz - 1);
+        const int nxy = g.nx * g.ny;
+        const int dslot = bd * kWave + lane;
+        const int dlin = cdi + g.nx * cdj + nxy * cdk;
+        const int bxi = bs % g.bx, byi = (bs / g.bx) % g.by, bzi = bs / (g.bx * g.by);
+        for (int q = 0; q < kWave; ++q) {
+            const int si = bxi * kBrick + (q & 3), sj = byi * kBrick + ((q >> 2) & 3), sk = bzi * kBrick + (q >> 4);
+            if (si >= g.nx || sj >= g.ny || sk >= g.nz) continue;  // wave-uniform
+            const int sslot = bs * kWave + q;
+            const int ix = cdi - si, iy = cdj - sj, iz = cdk - sk;
+            const T ddx = T(ix) * g.dx, ddy = T(iy) * g.dy, ddz = T(iz) * g.dz;
+            const T r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            const bool self = (ix == 0) && (iy == 0) && (iz == 0);
+            const bool once = bs != bd || sslot < dslot;
+            if (!(dvalid && !self && once && r2 < thr2)) continue;
+            const int slin = si + g.nx * sj + nxy * sk;
+            TA sw[NC];
+            bool any = false;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                sw[c] = B.scale * (TA(B.pairbuf[c][(size_t)slin * g.npad + dslot]) +
+                                   TA(B.pairbuf[c][(size_t)dlin * g.npad + sslot]));
+                any = any || sw[c] != TA(0);
+            }
+            if (!any) continue;
+            int cur = -1;
+            TA run = TA(0);
+            auto flush = [&]() {
+                if (cur >= 0) {
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) atomicAdd(&sacc[c * B.nparam + cur], sw[c] * run);
+                }
+            };
+            march_visit<T>(slin, ix, iy, iz, g.nx, nxy, sqrt(r2), [&](int v, T len) {
+                const int p = spart[v];
+                if (p != cur) {
+                    flush();
+                    cur = p;
+                    run = TA(0);
+                }
+                run += TA(len);
+            });
+            flush();
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < NC * B.nparam; i += blockDim.x)
+        if (sacc[i] != TA(0)) atomicAdd(&B.out[i / B.nparam][i % B.nparam], sacc[i]);
+}
+
+// ---------------------------------------------------------------------------
 // Source and ballistic derivatives along each beam column (column products of
 // k_source): s0_l0(k) = F_l h(k), h(k) = (mu_s/mu_t)(k) T(k) (1 - a_k),
 // a_k = e^{-mu_t(k) dz}, T(k) = prod_{k'<k} a_k', ballistic = fact T(nz).
@@ -601,6 +681,37 @@ hipError_t launch_backproject(const GridDev<T> &g, int nchunk, int chunk_len, T 
     return hipGetLastError();
 }
 
+template <typename T, typename TA>
+hipError_t launch_backproject_blocks(const GridDev<T> &g, T thr2, const BlockBp<T, TA> &B, int nc, hipStream_t st)
+{
+    static int lds_max = -1;
+    if (lds_max < 0) {
+        int dev = 0, v = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+        if (e != hipSuccess) return e;
+        lds_max = v;
+    }
+    if (nc < 1 || nc > kMaxNC || B.nparam < 1) return hipErrorInvalidValue;
+    const size_t lds = (((size_t)g.N * sizeof(int) + 15) & ~size_t(15)) + (size_t)nc * B.nparam * sizeof(TA);
+    if (lds > (size_t)lds_max) return hipErrorInvalidValue;  // the caller back-projects per voxel
+    const int ntiles = g.nblk * (g.nblk + 1) / 2;
+    constexpr int threads = 1024, waves = threads / kWave;
+    const void *fn = nc == 1 ? reinterpret_cast<const void *>(&k_backproject_blocks<T, TA, 1>)
+                             : reinterpret_cast<const void *>(&k_backproject_blocks<T, TA, 2>);
+    if (lds > 65536) {
+        const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    if (nc == 1)
+        hipLaunchKernelGGL((k_backproject_blocks<T, TA, 1>), dim3((ntiles + waves - 1) / waves), dim3(threads), lds,
+                           st, g, ntiles, thr2, B);
+    else
+        hipLaunchKernelGGL((k_backproject_blocks<T, TA, 2>), dim3((ntiles + waves - 1) / waves), dim3(threads), lds,
+                           st, g, ntiles, thr2, B);
+    return hipGetLastError();
+}
+
 template <typename TA>
 hipError_t launch_src_grad(int L, int nx, int ny, int nz, double dx, double dy, double dz, double xmin, double ymin,
                            double gan, const BeamDev &B, const TA *lam0, const TA *cot, TA *ga, TA *gs, hipStream_t st)
@@ -657,6 +768,10 @@ template hipError_t launch_backproject<float, double>(const GridDev<float> &, in
                                                       double, double *, hipStream_t);
 template hipError_t launch_backproject<double, double>(const GridDev<double> &, int, int, double, const double *,
                                                        double, double *, hipStream_t);
+template hipError_t launch_backproject_blocks<float, double>(const GridDev<float> &, float,
+                                                             const BlockBp<float, double> &, int, hipStream_t);
+template hipError_t launch_backproject_blocks<double, double>(const GridDev<double> &, double,
+                                                              const BlockBp<double, double> &, int, hipStream_t);
 template hipError_t launch_src_grad<double>(int, int, int, int, double, double, double, double, double, double,
                                             const BeamDev &, const double *, const double *, double *, double *,
                                             hipStream_t);

@@ -1435,7 +1435,7 @@ struct AdjointRun {
     HostSolve h{};
     SolveDev<T, TA> sv{};
     std::vector<TA> E;
-    DevBuf dW, dWp, dS0, dPart, dPix, dRed, dSv, dBeam, dCoefE, dWq, dWqz;
+    DevBuf dW, dWp, dS0, dPart, dPix, dRed, dSv, dBeam, dCoefE, dWq, dWqz, dBp;
     // per cotangent of a backward pass (up to kMaxNC share its propagation launches;
     // the first one's chunk partials reuse the forward's dPart)
     DevBuf dCot[kMaxNC], dX[kMaxNC], dXz[kMaxNC], dPartC[kMaxNC], dPair[kMaxNC], dGt[kMaxNC], dGs[kMaxNC],
@@ -1599,9 +1599,21 @@ struct AdjointRun {
         return RTE_OK;
     }
 
+    // Optional output of a backward pass: the back-projection's mu_t gradient summed
+    // over parameter blocks (part: device [N], block of each voxel or -1) into bp[c]
+    // (host [nparam]) instead of per voxel.  used reports whether it was taken (the
+    // blocks' accumulators fit in LDS); then dmu_a and dmu_s leave that part out.
+    struct BlockOut {
+        const int *part = nullptr;
+        int nparam = 0;
+        double *bp[kMaxNC] = {};
+        bool used = false;
+    };
+
     // d(cot . E)/d mu_a, d mu_s for every voxel, for nc <= kMaxNC cotangents at once:
     // their adjoint series share every propagation launch
-    int backward_multi(int nc, const double *const *cot_host, double *const *dmu_a, double *const *dmu_s)
+    int backward_multi(int nc, const double *const *cot_host, double *const *dmu_a, double *const *dmu_s,
+                       BlockOut *blk = nullptr)
     {
         if (nc < 1 || nc > kMaxNC) return fail(RTE_E_ARG, "backward: %d cotangents per pass (1..%d)", nc, kMaxNC);
         int rc;
@@ -1688,8 +1700,34 @@ struct AdjointRun {
             }
         }
         std::vector<TA> gt(N), gs(N), ga2(N), gs2(N);
+        if (blk) {
+            blk->used = false;
+            BlockBp<T, TA> B{};
+            B.part = blk->part;
+            B.nparam = blk->nparam;
+            B.scale = TA(-dV);
+            HIP_TRY(dBp.alloc(sizeof(TA) * kMaxNC * (size_t)blk->nparam));
+            HIP_TRY(launch_fill<TA>(dBp.as<TA>(), (size_t)kMaxNC * blk->nparam, TA(0), st));
+            for (int c = 0; c < nc; ++c) {
+                B.pairbuf[c] = dPair[c].as<T>();
+                B.out[c] = dBp.as<TA>() + (size_t)c * blk->nparam;
+            }
+            const hipError_t e = launch_backproject_blocks<T, TA>(g, pa.thr2, B, nc, st);
+            if (e == hipSuccess) {
+                std::vector<TA> bp((size_t)nc * blk->nparam);
+                HIP_TRY(hipMemcpyAsync(bp.data(), dBp.p, sizeof(TA) * bp.size(), hipMemcpyDeviceToHost, st));
+                HIP_TRY(hipStreamSynchronize(st));
+                for (int c = 0; c < nc; ++c)
+                    for (int q = 0; q < blk->nparam; ++q) blk->bp[c][q] = (double)bp[(size_t)c * blk->nparam + q];
+                blk->used = true;
+            } else if (e != hipErrorInvalidValue) {
+                HIP_TRY(e);
+            }
+        }
         for (int c = 0; c < nc; ++c) {
-            HIP_TRY(launch_backproject<T, TA>(g, nch, clen, pa.thr2, dPair[c].as<T>(), TA(-dV), dGt[c].as<TA>(), st));
+            if (!(blk && blk->used))
+                HIP_TRY(launch_backproject<T, TA>(g, nch, clen, pa.thr2, dPair[c].as<T>(), TA(-dV), dGt[c].as<TA>(),
+                                                  st));
             HIP_TRY(launch_src_grad<TA>(L, grid.nx, grid.ny, grid.nz, dx, dy, dz, grid.xmin, grid.ymin, m->g, bd,
                                         dLam0[c].as<TA>(), dCot[c].as<TA>(), dGa2[c].as<TA>(), dGs2[c].as<TA>(), st));
             HIP_TRY(hipMemcpyAsync(gt.data(), dGt[c].p, sizeof(TA) * N, hipMemcpyDeviceToHost, st));
@@ -1875,7 +1913,14 @@ static int fisher_run(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const r
             if (E[pix] != 0.0) pixels.push_back(pix);
         }
         std::vector<std::vector<double>> cot(kMaxNC, std::vector<double>(P, 0.0)), ga(kMaxNC, std::vector<double>(N)),
-            gs(kMaxNC, std::vector<double>(N));
+            gs(kMaxNC, std::vector<double>(N)), bp(kMaxNC, std::vector<double>(nparam));
+        // the back-projection straight into the blocks (one march for both passes)
+        DevBuf dpart;
+        HIP_TRY(dpart.alloc(sizeof(int) * N));
+        HIP_TRY(hipMemcpy(dpart.p, part, sizeof(int) * N, hipMemcpyHostToDevice));
+        typename AdjointRun<T, TA>::BlockOut blk;
+        blk.part = dpart.as<int>();
+        blk.nparam = nparam;
         for (size_t o = 0; o < pixels.size(); o += kMaxNC) {
             const int nc = (int)std::min<size_t>(kMaxNC, pixels.size() - o);
             const double *cp[kMaxNC];
@@ -1886,13 +1931,16 @@ static int fisher_run(rte_ctx *ctx, rte_medium *m, const rte_beam *beam, const r
                 cp[c] = cot[c].data();
                 ap[c] = ga[c].data();
                 sp[c] = gs[c].data();
+                blk.bp[c] = bp[c].data();
             }
-            if ((rc = run.backward_multi(nc, cp, ap, sp))) return rc;
+            if ((rc = run.backward_multi(nc, cp, ap, sp, &blk))) return rc;
             for (int c = 0; c < nc; ++c) {
                 const std::vector<double> &gv = wrt == 0 ? ga[c] : gs[c];
                 const int pix = pixels[o + c];
                 for (int v = 0; v < N; ++v)
                     if (part[v] >= 0) J[(size_t)pix * nparam + part[v]] += gv[v];
+                if (blk.used)
+                    for (int q = 0; q < nparam; ++q) J[(size_t)pix * nparam + q] += bp[c][q];
             }
         }
         run.fill_stats(stats);

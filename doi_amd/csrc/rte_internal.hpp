@@ -212,6 +212,17 @@ hipError_t launch_accumulate(size_t n, const TA *x, TA *y, hipStream_t st);
 template <typename T, typename TA>
 hipError_t launch_make_wq(int L, const GridDev<T> &g, const Scales &sc, TA gan, const T *mus, const TA *aw_prev,
                           T *Wq, hipStream_t st);
+// back-projection onto parameter blocks for up to kMaxNC backward passes
+template <typename T, typename TA>
+struct BlockBp {
+    const T *pairbuf[kMaxNC];  // [N][npad] pair weights of each pass
+    const int *part;           // [N] block of each voxel, -1 for none
+    int nparam;
+    TA scale;
+    TA *out[kMaxNC];           // [nparam] block sums, added to
+};
+template <typename T, typename TA>
+hipError_t launch_backproject_blocks(const GridDev<T> &g, T thr2, const BlockBp<T, TA> &B, int nc, hipStream_t st);
 template <typename T, typename TA>
 hipError_t launch_backproject(const GridDev<T> &g, int nchunk, int chunk_len, T thr2, const T *pairbuf, TA scale,
                               TA *gt, hipStream_t st);

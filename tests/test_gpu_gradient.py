@@ -147,6 +147,36 @@ def test_fisher_information(ctx):
     assert np.max(np.abs(Cf @ Cf.T - F)) < 1e-12 * np.max(np.abs(F))
 
 
+@pytest.mark.parametrize("wrt", [0, 1])
+def test_exact_jacobian_rows_are_block_sums_of_the_vjp(ctx, wrt):
+    """The exact Fisher path back-projects onto the parameter blocks directly, two
+    detectors per backward pass; each row of its J must equal the per-voxel
+    vector-Jacobian product of that detector summed over the blocks (FP64, a
+    partition with voxels outside every block; 25 detectors, so the last pass has one)."""
+    shape = (5, 5, 4)
+    bounds = bounds_for(shape)
+    ma, ms = hetero_maps(shape, seed=23, sigma=0.3)
+    i, j, k = np.meshgrid(np.arange(5), np.arange(5), np.arange(4), indexing="ij")
+    part = ((i // 2) + 3 * (j // 2) + 9 * (k // 2)).transpose(2, 1, 0).ravel()
+    part[(np.arange(part.size) % 7) == 3] = -1
+    nparam = int(part.max()) + 1
+    beam = (0.0, 0.0, 0.1, 1.0)  # on the centre column of the odd grid
+    med = ctx.medium(shape, bounds, ma, ms, G)
+    p = rte.make_params(L, precision=rte.PREC_FP64, stop_rule=rte.STOP_DELTA, tol=1e-8, max_terms=40)
+    _, _, J, img, _ = ctx.fisher(med, beam, p, part, nparam, wrt=wrt, jacobian=0)
+    E = img.ravel()
+    for pix in (0, 7, 12, 24):
+        if E[pix] == 0:  # a detector the Jacobian skips
+            continue
+        cot = np.zeros(E.size)
+        cot[pix] = 1.0
+        _, ga, gs, _ = ctx.exitance_vjp(med, beam, p, cot)
+        g = ga if wrt == 0 else gs
+        row = np.array([g[part == a].sum() for a in range(nparam)])
+        assert np.max(np.abs(J[pix] - row)) < 1e-10 * np.max(np.abs(row)), pix
+    med.close()
+
+
 @pytest.mark.parametrize("nref", [1.0, 1.37])
 def test_reference_jacobian_matches_oracle(ctx, nref):
     """rte_ref_jacobian (the reference's own forward-mode derivative) against the
