@@ -1646,9 +1646,10 @@ struct AdjointRun {
             A.dV = ca.dV;
             for (int l = 0; l <= kMaxL; ++l) A.coefE[l] = ca.coefE[l];
             for (int a = 0; a < kMaxK; ++a) A.nsc[a] = ca.nsc[a];
+            partc[c] = c == 0 ? dPart.as<T>() : dPartC[c].as<T>();
             A.X = dX[c].as<T>();
             A.Wq = dWq.as<T>();
-            A.part = c == 0 ? dPart.as<T>() : dPartC[c].as<T>();
+            A.part = partc[c];
             A.mut = sv.mut;
             A.mus = sv.mus;
             A.g_aniso = gan;
@@ -1657,7 +1658,6 @@ struct AdjointRun {
             A.gs = dGs[c].as<TA>();
             Xc[c] = dX[c].as<T>();
             Xzc[c] = dXz[c].as<T>();
-            partc[c] = const_cast<T *>(A.part);
             pairc[c] = dPair[c].as<T>();
         }
         SolveDev<T, TA> *svd = dSv.as<SolveDev<T, TA>>();

@@ -40,7 +40,7 @@ __global__ void k_fill(T *__restrict__ p, size_t n, T v)
 //   workgroup share one CU and walk the same sources of the same solves at
 //   the same time, so one scalar-cache fill of a W row serves all of them.
 //   Group z holds the NR solves sv[z*NR .. z*NR+NR-1], which share one medium
-//   (same path table and mu_t): the pair direction, its SH recurrences and kf
+//   (same path table and mu_t): the pair direction, its z-power values and kf
 //   are computed once and applied to every right-hand side.
 // PATH 0: kf from the per-medium table; PATH 1: marched per pair.
 // ---------------------------------------------------------------------------
@@ -205,7 +205,7 @@ __global__ __launch_bounds__(kWave * kPropWaves) void k_prop(PropArgs<T> A, cons
             T c[H];
             T p, q, t;
             cptr<T> wp;
-            // the harmonics complete before the first projection, so the wait for
+            // the pair values complete before the first projection, so the wait for
             // this source's first row (requested during the last side of the
             // previous source) comes after them
             __builtin_amdgcn_sched_barrier(0);
@@ -261,7 +261,7 @@ __global__ __launch_bounds__(kWave * kPropWaves) void k_prop(PropArgs<T> A, cons
             pair_axpy_z<T, L>(acc[2], t, G);
             __builtin_amdgcn_sched_barrier(0);
         } else {
-            // the harmonics complete before the first projection: interleaved with
+            // the pair values complete before the first projection: interleaved with
             // it, they pulled the wait for the row requested during the previous
             // source's last axpy up to 69 VALU instructions after the request (120 now)
             __builtin_amdgcn_sched_barrier(0);
