@@ -330,8 +330,9 @@ __global__ __launch_bounds__(256) void k_to_zpow(int N, const SolveDev<T, TA> *_
 // bs <= bd once (from s to d) and writes both entries: the direct one
 // (row s, the wave's 64 destination slots: one coalesced row segment per
 // source) and, off the diagonal, the mirrored one (row d, the source brick's
-// slots), staged one x row of 4 sources at a time in registers and written
-// as a 16-byte run per lane.  mu_t is read from LDS when the whole map fits.
+// slots), held in registers and written as one 256-byte run per lane after the
+// brick's march (FP32; FP64 one x row of 4 sources at a time).  mu_t is read
+// from LDS when the whole map fits.
 // grid = ceil(ntiles / waves per block), block = KT threads.
 // ---------------------------------------------------------------------------
 template <typename T, bool LDS>
@@ -361,44 +362,69 @@ __global__ __launch_bounds__(1024) void k_ktab(GridDev<T> g, int ntiles, T thr2,
     const int dlin = cdi + g.nx * cdj + nxy * cdk;
     const int bxi = bs % g.bx, byi = (bs / g.bx) % g.by, bzi = bs / (g.bx * g.by);
     const bool mirror = bs != bd;
-    for (int q = 0; q < 16; ++q) {  // the source brick's x rows, 4 voxels each
-        T m[4];
-#pragma unroll
-        for (int jj = 0; jj < 4; ++jj) {
-            const int si = bxi * kBrick + jj, sj = byi * kBrick + (q & 3), sk = bzi * kBrick + (q >> 2);
-            T kf = T(0);
-            if (si < g.nx && sj < g.ny && sk < g.nz) {  // wave-uniform
-                const int ix = cdi - si, iy = cdj - sj, iz = cdk - sk;
-                const T ddx = T(ix) * g.dx, ddy = T(iy) * g.dy, ddz = T(iz) * g.dz;
-                const T r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                const bool self = (ix == 0) && (iy == 0) && (iz == 0);
-                const bool use = dvalid && !self && (r2 < thr2);
-                const T r = sqrt(self ? T(1) : r2);
-                const int slin = si + g.nx * sj + nxy * sk;
-                T tau = T(0);
-                if (use) {
-                    if constexpr (LDS)
-                        tau = march_tau<T>((const __attribute__((address_space(3))) T *)ktab_lds, slin, ix, iy, iz,
-                                           g.nx, nxy, r);
-                    else
-                        tau = march_tau<T>(as_global<T>(mut), slin, ix, iy, iz, g.nx, nxy, r);
-                }
-                kf = use ? dev_exp<T>(-tau) / r2 : T(0);
-                ktab[(size_t)slin * g.npad + dslot] = kf;
+    // kf of source (si, sj, sk) of the brick for this lane's destination, written
+    // to the direct entry (row s, this wave's 64 slots: one coalesced segment)
+    auto entry = [&](int jj, int q) -> T {
+        const int si = bxi * kBrick + jj, sj = byi * kBrick + (q & 3), sk = bzi * kBrick + (q >> 2);
+        T kf = T(0);
+        if (si < g.nx && sj < g.ny && sk < g.nz) {  // wave-uniform
+            const int ix = cdi - si, iy = cdj - sj, iz = cdk - sk;
+            const T ddx = T(ix) * g.dx, ddy = T(iy) * g.dy, ddz = T(iz) * g.dz;
+            const T r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            const bool self = (ix == 0) && (iy == 0) && (iz == 0);
+            const bool use = dvalid && !self && (r2 < thr2);
+            const T r = sqrt(self ? T(1) : r2);
+            const int slin = si + g.nx * sj + nxy * sk;
+            T tau = T(0);
+            if (use) {
+                if constexpr (LDS)
+                    tau = march_tau<T>((const __attribute__((address_space(3))) T *)ktab_lds, slin, ix, iy, iz, g.nx,
+                                       nxy, r);
+                else
+                    tau = march_tau<T>(as_global<T>(mut), slin, ix, iy, iz, g.nx, nxy, r);
             }
-            m[jj] = kf;
+            kf = use ? dev_exp<T>(-tau) / r2 : T(0);
+            ktab[(size_t)slin * g.npad + dslot] = kf;
         }
-        if (mirror && dvalid) {
-            using V = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
-            constexpr int per = sizeof(V) / sizeof(T);
-            V *row = reinterpret_cast<V *>(ktab + (size_t)dlin * g.npad + bs * kWave + q * 4);
+        return kf;
+    };
+    using V = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+    constexpr int per = sizeof(V) / sizeof(T);
+    V *row = reinterpret_cast<V *>(ktab + (size_t)dlin * g.npad + bs * kWave);
+    if constexpr (sizeof(T) == 4) {
+        // FP32: the brick's 64 mirrored entries stay in registers and go out as one
+        // 256-byte run per lane after the march (written piecewise, the rows' lines
+        // reached HBM partly filled: 2.1x the table's bytes)
+        T m[kWave];
 #pragma unroll
-            for (int u = 0; u < 4 / per; ++u) {
+        for (int q = 0; q < 16; ++q)  // the source brick's x rows, 4 voxels each
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) m[q * 4 + jj] = entry(jj, q);
+        if (mirror && dvalid) {
+#pragma unroll
+            for (int u = 0; u < kWave / per; ++u) {
                 V x;
                 T *xs = reinterpret_cast<T *>(&x);
 #pragma unroll
                 for (int e = 0; e < per; ++e) xs[e] = m[u * per + e];
                 row[u] = x;
+            }
+        }
+    } else {
+        // FP64: one x row of 4 sources at a time (the whole brick would spill)
+        for (int q = 0; q < 16; ++q) {
+            T m[4];
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) m[jj] = entry(jj, q);
+            if (mirror && dvalid) {
+#pragma unroll
+                for (int u = 0; u < 4 / per; ++u) {
+                    V x;
+                    T *xs = reinterpret_cast<T *>(&x);
+#pragma unroll
+                    for (int e = 0; e < per; ++e) xs[e] = m[u * per + e];
+                    row[q * 4 / per + u] = x;
+                }
             }
         }
     }
