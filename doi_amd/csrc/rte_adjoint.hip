@@ -87,7 +87,7 @@ __global__ __launch_bounds__(kWave) void k_prop_adj(PropArgs<T> A, AdjDev<T, TA>
     // the table entry and the pair weights of the next source are loaded one source
     // ahead (the last source's again at the chunk's end: in bounds, unused); a lane
     // outside the grid keeps its weights in registers and never stores them
-    T kf_next = s0 < s1 ? ktab[(size_t)s0 * g.npad + slot] : T(0), pw_next[NC];
+    T kf_next = s0 < s1 ? __builtin_nontemporal_load(ktab + (size_t)s0 * g.npad + slot) : T(0), pw_next[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) pw_next[c] = s0 < s1 ? pb[c][(size_t)s0 * g.npad] : T(0);
     for (int s = s0; s < s1; ++s) {
@@ -97,7 +97,7 @@ __global__ __launch_bounds__(kWave) void k_prop_adj(PropArgs<T> A, AdjDev<T, TA>
         const T kf = kf_next;
         T pw[NC];
         const int sn = (s + 1 < s1) ? s + 1 : s;
-        kf_next = ktab[(size_t)sn * g.npad + slot];
+        kf_next = __builtin_nontemporal_load(ktab + (size_t)sn * g.npad + slot);  // read once, as in k_prop
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
             pw[c] = pw_next[c];

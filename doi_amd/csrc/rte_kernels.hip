@@ -92,7 +92,7 @@ __global__ __launch_bounds__(kWave * kPropWaves) void k_prop(PropArgs<T> A, cons
     if constexpr (PATH == 0) {
 #pragma unroll
         for (int r = 0; r < NT; ++r)
-            if (s0 < s1) kf_next[r] = ktab[r][(size_t)s0 * g.npad + slot];
+            if (s0 < s1) kf_next[r] = __builtin_nontemporal_load(ktab[r] + (size_t)s0 * g.npad + slot);
     }
     // FP64 complex group: the pair phase (cos, sin)(kim r) from the per-displacement
     // table (read where it is used: requested a source ahead, the two values live
@@ -112,12 +112,14 @@ __global__ __launch_bounds__(kWave * kPropWaves) void k_prop(PropArgs<T> A, cons
             // the self pair's table entry is 0 and its direction (0, 0, 0) whatever r is:
             // clamping r^2 keeps rsqrt finite without testing the indices
             r2 = max(r2, T(1e-30));
-            // the next source's entry (the last source's again at the end: in bounds, unused)
+            // the next source's entry (the last source's again at the end: in bounds, unused).
+            // Each entry is read once per launch and group: a nontemporal load, so the
+            // table's stream does not push the W rows, which every brick reads, out of L2
             const int sn = (s + 1 < s1) ? s + 1 : s;
 #pragma unroll
             for (int r = 0; r < NT; ++r) {
                 kfs[r] = kf_next[r];
-                kf_next[r] = ktab[r][(size_t)sn * g.npad + slot];
+                kf_next[r] = __builtin_nontemporal_load(ktab[r] + (size_t)sn * g.npad + slot);
             }
         } else {
             const bool self = (cdi == si) && (cdj == sj) && (cdk == sk);
