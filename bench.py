@@ -31,7 +31,7 @@ PEAK_HBM_GBS = 8000.0
 # FETCH_SIZE + WRITE_SIZE calibrated against a device copy: a beam in a group of three
 # and of two at the benchmark's full size (profiles/k_prop_32cube_L5_bench_hbm_traffic.md),
 # alone on 4 media (profiles/k_prop_32cube_L5_4media_hbm_traffic.md)
-TRAFFIC_PER_SOLVE_TERM = {3: 1.567e9, 2: 2.28e9, 1: 4.41e9}
+TRAFFIC_PER_SOLVE_TERM = {3: 1.508e9, 2: 2.273e9, 1: 4.41e9}
 DEFAULT_GROUP = 3          # the library's default number of beams per propagation group
 
 

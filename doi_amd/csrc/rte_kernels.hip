@@ -104,6 +104,19 @@ __global__ __launch_bounds__(kWave * kPropWaves) void k_prop(PropArgs<T> A, cons
     T w[K];
 #pragma unroll
     for (int a = 0; a < K; ++a) w[a] = (s0 < s1) ? Wsrc[0][(size_t)s0 * K + a] : T(0);
+    // The W rows of a chunk's sources are read by every brick, but through the scalar
+    // cache, whose misses wait for all scalar loads in flight; fetched from beyond L2
+    // they stalled the group of three's waves.  So each of its waves also touches its
+    // rows PFD sources ahead with one vector load (lane l < 4 NR: float {0, 12, 24, 35}
+    // [l % 4] of right-hand side l / 4's row, which reaches every 64-byte line of it),
+    // bringing them into L2; the loaded values are kept alive, unused.  The group of
+    // three on one medium only: the group of two (which waited little) and single beams
+    // gained nothing from it, and the separate-media and complex groups would lose a
+    // wave per SIMD to the two address registers.
+    constexpr bool PF = PATH == 0 && !SEP && !CPLX && sizeof(T) == 4 && NR == 3;
+    constexpr int PFD = 2;
+    const gptr<T> pfrow = as_global<T>(grp[min(lane / 4, NR - 1)].Wp + min((lane % 4) * 12, K - 1));
+    T pf_keep = T(0);
     for (int s = s0; s < s1; ++s) {
         const T ddx = fmad(-T(si), g.dx, dxc), ddy = fmad(-T(sj), g.dy, dyc), ddz = fmad(-T(sk), g.dz, dzc);
         T r2 = fmad(ddx, ddx, fmad(ddy, ddy, ddz * ddz));
@@ -131,6 +144,8 @@ __global__ __launch_bounds__(kWave * kPropWaves) void k_prop(PropArgs<T> A, cons
             r2 = self ? T(1) : r2;
         }
         const T kf = kfs[0];
+        T pfx = T(0);
+        if constexpr (PF) pfx = pfrow[(size_t)((s + PFD < s1) ? s + PFD : s1 - 1) * K];
         const T rinv = dev_rsqrt<T>(r2);
         // the pair's Z values, formed once and shared by every right-hand side
         PairZ<T, L> G;
@@ -283,6 +298,7 @@ __global__ __launch_bounds__(kWave * kPropWaves) void k_prop(PropArgs<T> A, cons
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
+        if constexpr (PF) pf_keep += pfx;  // at the source's end: the load is waited for there
         if (++si == g.nx) {
             si = 0;
             if (++sj == g.ny) {
@@ -291,6 +307,7 @@ __global__ __launch_bounds__(kWave * kPropWaves) void k_prop(PropArgs<T> A, cons
             }
         }
     }
+    asm volatile("" ::"v"(pf_keep));
     if (valid) {
         // partial sums in the Z basis (k_combine turns their chunk sum back)
         const int v = di + g.nx * (dj + g.ny * dk);
