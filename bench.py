@@ -28,11 +28,12 @@ sys.path.insert(0, ROOT)
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 (vector/matrix dense) published peak
 PEAK_HBM_GBS = 8000.0
 # HBM bytes per solve and term of k_prop at 32^3 L=5 (mixed, cached), from rocprofv3
-# FETCH_SIZE + WRITE_SIZE calibrated against a device copy: a beam in a group of three
-# and of two at the benchmark's full size (profiles/k_prop_32cube_L5_bench_hbm_traffic.md),
-# alone on 4 media (profiles/k_prop_32cube_L5_4media_hbm_traffic.md)
-TRAFFIC_PER_SOLVE_TERM = {3: 1.508e9, 2: 2.273e9, 1: 4.41e9}
-DEFAULT_GROUP = 3          # the library's default number of beams per propagation group
+# FETCH_SIZE + WRITE_SIZE calibrated against a device copy: a beam in a group of five,
+# three and two at the benchmark's full size
+# (profiles/k_prop_32cube_L5_bench_hbm_traffic.md), alone on 4 media
+# (profiles/k_prop_32cube_L5_4media_hbm_traffic.md)
+TRAFFIC_PER_SOLVE_TERM = {5: 0.960e9, 3: 1.508e9, 2: 2.273e9, 1: 4.41e9}
+DEFAULT_GROUP = 5          # the library's default number of beams per propagation group (L <= 5)
 
 
 def log(*a):
@@ -596,8 +597,9 @@ def main():
     terms_mean = float(np.mean([s["terms"] for s in stats]))
 
     # roofline of the dominant kernel, the propagation group of the default size
-    # (k_prop<float, double, L, 0, 3>): its algorithmic flops / its launch time
-    gsz = args.group or DEFAULT_GROUP
+    # (k_prop<float, double, L, 0, 5>; the library allows at most 3 above L = 5):
+    # its algorithmic flops / its launch time
+    gsz = min(args.group or DEFAULT_GROUP, args.beams, 5 if L <= 5 else 3)
     launches = max(prof["group_launches"][gsz], 1)
     avg_ms = prof["group_ms"][gsz] / launches
     flops_per_launch = prof["group_flops"][gsz] / launches

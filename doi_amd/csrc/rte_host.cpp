@@ -308,7 +308,7 @@ struct rte_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool prof = false;
-    int max_rhs = 3;  // beams per propagation group (1..kMaxNR); 3 measured fastest at 32^3 L=5
+    int max_rhs = 5;  // beams per propagation group (1..kMaxNR); 5 measured fastest at 32^3 L=5
     rte_prof pc{};
     std::vector<hipEvent_t> ev_pool;
     std::vector<PendingTiming> pending;
@@ -1008,7 +1008,8 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
                 std::vector<int> same;
                 for (int q = j; q < na; ++q)
                     if (!taken[q] && svact[q].ktab == svact[j].ktab) same.push_back(q);
-                const size_t gs = (size_t)ctx->max_rhs;
+                // above L = 5 a group's accumulators outgrow the register file: at most 3
+                const size_t gs = (size_t)std::min(ctx->max_rhs, L <= 5 ? kMaxNR : 3);
                 for (size_t o = 0; o < same.size(); o += gs) {
                     const size_t e = std::min(same.size(), o + gs);
                     for (size_t q = o; q < e; ++q) {

@@ -106,14 +106,14 @@ __global__ __launch_bounds__(kWave * kPropWaves) void k_prop(PropArgs<T> A, cons
     for (int a = 0; a < K; ++a) w[a] = (s0 < s1) ? Wsrc[0][(size_t)s0 * K + a] : T(0);
     // The W rows of a chunk's sources are read by every brick, but through the scalar
     // cache, whose misses wait for all scalar loads in flight; fetched from beyond L2
-    // they stalled the group of three's waves.  So each of its waves also touches its
-    // rows PFD sources ahead with one vector load (lane l < 4 NR: float {0, 12, 24, 35}
-    // [l % 4] of right-hand side l / 4's row, which reaches every 64-byte line of it),
-    // bringing them into L2; the loaded values are kept alive, unused.  The group of
-    // three on one medium only: the group of two (which waited little) and single beams
-    // gained nothing from it, and the separate-media and complex groups would lose a
-    // wave per SIMD to the two address registers.
-    constexpr bool PF = PATH == 0 && !SEP && !CPLX && sizeof(T) == 4 && NR == 3;
+    // they stalled the waves of the groups of three to five.  So each of their waves also
+    // touches its rows PFD sources ahead with one vector load (lane l < 4 NR: float
+    // {0, 12, 24, 35}[l % 4] of right-hand side l / 4's row, which reaches every 64-byte
+    // line of it), bringing them into L2; the loaded values are kept alive, unused.
+    // Groups of three or more on one medium only: the group of two (which waited
+    // little) and single beams gained nothing from it, and the separate-media and
+    // complex groups would lose a wave per SIMD to the two address registers.
+    constexpr bool PF = PATH == 0 && !SEP && !CPLX && sizeof(T) == 4 && NR >= 3;
     constexpr int PFD = 2;
     const gptr<T> pfrow = as_global<T>(grp[min(lane / 4, NR - 1)].Wp + min((lane % 4) * 12, K - 1));
     T pf_keep = T(0);

@@ -55,9 +55,9 @@ def test_32cube_mixed_within_1e4_of_fp64(ctx):
 def test_32cube_five_beams_default_groups(ctx):
     """The benchmark's own propagation instance: five beams on one 32^3 L=5
     medium of the benchmark's media, mixed precision, default group size (one
-    group of three and one of two right-hand sides), tolerance 1e-6 with a cap
-    of 30 terms.  Same term counts as the all-FP64 solves, within 1e-4 of
-    them, and the same bits as each beam solved alone."""
+    group of five right-hand sides) and group size 3 (groups of three and two),
+    tolerance 1e-6 with a cap of 30 terms.  Same term counts as the all-FP64
+    solves, within 1e-4 of them, and the same bits as each beam solved alone."""
     n, L = 32, 5
     ma, ms = media.lognormal_media_with_inclusions(1, seed=5, n=n)[0]
     m = ctx.medium((n, n, n), media.cube_bounds(n), ma, ms, 0.9)
@@ -65,15 +65,22 @@ def test_32cube_five_beams_default_groups(ctx):
     assert len(beams) == 5
     pm = rte.make_params(L, precision=rte.PREC_MIXED, stop_rule=rte.STOP_DELTA, tol=1e-6, max_terms=30)
     p64 = rte.make_params(L, precision=rte.PREC_FP64, stop_rule=rte.STOP_DELTA, tol=1e-6, max_terms=30)
-    print("32^3 five beams, groups of 3 + 2", flush=True)
+    print("32^3 five beams, one group of 5", flush=True)
     batch, st = ctx.solve_media([m] * 5, beams, pm)
+    print("32^3 five beams, groups of 3 + 2", flush=True)
+    try:
+        ctx.set_group_size(3)
+        batch32, st32 = ctx.solve_media([m] * 5, beams, pm)
+    finally:
+        ctx.set_group_size(5)
     print("32^3 five beams FP64", flush=True)
     ref, st64 = ctx.solve_media([m] * 5, beams, p64)
     print("32^3 five beams alone", flush=True)
     for b in range(5):
         alone, sta = ctx.solve_media([m], [beams[b]], pm)
-        assert sta[0]["terms"] == st[b]["terms"] == st64[b]["terms"], b
+        assert sta[0]["terms"] == st[b]["terms"] == st32[b]["terms"] == st64[b]["terms"], b
         assert np.array_equal(alone[0], batch[b]), b
+        assert np.array_equal(alone[0], batch32[b]), b
         assert rel_max(batch[b], ref[b]) < 1e-4, b
     m.close()
 

@@ -149,15 +149,16 @@ def test_batch_equals_alone(ctx):
         m.close()
 
 
-DEFAULT_GROUP = 3  # rte_ctx_set_group_size's default
+DEFAULT_GROUP = 5  # rte_ctx_set_group_size's default
 
 
 @pytest.mark.parametrize("L", [3, 4, 5])
 def test_beams_sharing_a_medium_equal_alone(ctx, L):
-    """Seven beams on one medium, propagated as groups of 5 and 2 (group size 5),
-    3, 3 and 1 (group size 3, the default) or 2, 2, 2 and 1 (group size 2)
-    right-hand sides, give the same bits as each beam solved alone (L 3 and 5:
-    the group of three reads W rows in halves; L 4: in whole rows)."""
+    """Seven beams on one medium, propagated as groups of 5 and 2 (group size 5,
+    the default), 4 and 3 (group size 4), 3, 3 and 1 (group size 3) or 2, 2, 2
+    and 1 (group size 2) right-hand sides, give the same bits as each beam solved
+    alone (L 3 and 5: the group of three reads W rows in halves; L 4: in whole
+    rows; groups of three to five prefetch their rows)."""
     shape = (12, 12, 12)
     bounds = bounds_for(shape)
     m = ctx.medium(shape, bounds, *hetero_maps(shape, seed=31), 0.9)
@@ -166,7 +167,7 @@ def test_beams_sharing_a_medium_equal_alone(ctx, L):
     p = rte.make_params(L, precision=rte.PREC_MIXED, stop_rule=rte.STOP_DELTA, tol=1e-6, max_terms=20)
     alone = [ctx.solve_media([m], [b], p) for b in beams]
     try:
-        for gsize in (5, DEFAULT_GROUP, 2):
+        for gsize in (5, 4, 3, 2):
             ctx.set_group_size(gsize)
             batch, st = ctx.solve_media([m] * 7, beams, p)
             for i in range(7):
