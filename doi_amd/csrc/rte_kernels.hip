@@ -115,7 +115,7 @@ __global__ __launch_bounds__(kWave * kPropWaves) void k_prop(PropArgs<T> A, cons
     // complex groups would lose a wave per SIMD to the two address registers.
     constexpr bool PF = PATH == 0 && !SEP && !CPLX && sizeof(T) == 4 && NR >= 3;
     constexpr int PFD = 2;
-    const gptr<T> pfrow = as_global<T>(grp[min(lane / 4, NR - 1)].Wp + min((lane % 4) * 12, K - 1));
+    const gptr<T> pfrow = PF ? as_global<T>(grp[min(lane / 4, NR - 1)].Wp + min((lane % 4) * 12, K - 1)) : nullptr;
     T pf_keep = T(0);
     for (int s = s0; s < s1; ++s) {
         const T ddx = fmad(-T(si), g.dx, dxc), ddy = fmad(-T(sj), g.dy, dyc), ddz = fmad(-T(sk), g.dz, dzc);
