@@ -597,9 +597,9 @@ def main():
     terms_mean = float(np.mean([s["terms"] for s in stats]))
 
     # roofline of the dominant kernel, the propagation group of the default size
-    # (k_prop<float, double, L, 0, 5>; the library allows at most 3 above L = 5):
-    # its algorithmic flops / its launch time
-    gsz = min(args.group or DEFAULT_GROUP, args.beams, 5 if L <= 5 else 3)
+    # (k_prop<float, double, L, 0, 5>; the library allows at most 3 at L = 6 and 2 from
+    # L = 7): its algorithmic flops / its launch time
+    gsz = min(args.group or DEFAULT_GROUP, args.beams, 5 if L <= 5 else (3 if L == 6 else 2))
     launches = max(prof["group_launches"][gsz], 1)
     avg_ms = prof["group_ms"][gsz] / launches
     flops_per_launch = prof["group_flops"][gsz] / launches

@@ -1008,8 +1008,9 @@ static int run_batch(rte_ctx *ctx, int nb, rte_medium *const *media, const rte_b
                 std::vector<int> same;
                 for (int q = j; q < na; ++q)
                     if (!taken[q] && svact[q].ktab == svact[j].ktab) same.push_back(q);
-                // above L = 5 a group's accumulators outgrow the register file: at most 3
-                const size_t gs = (size_t)std::min(ctx->max_rhs, L <= 5 ? kMaxNR : 3);
+                // above L = 5 a group's accumulators outgrow the register file: at most
+                // 3 at L = 6 and 2 from L = 7 (measured fastest at 24^3)
+                const size_t gs = (size_t)std::min(ctx->max_rhs, L <= 5 ? kMaxNR : (L == 6 ? 3 : 2));
                 for (size_t o = 0; o < same.size(); o += gs) {
                     const size_t e = std::min(same.size(), o + gs);
                     for (size_t q = o; q < e; ++q) {

@@ -135,9 +135,9 @@ typedef struct rte_medium rte_medium;
 int rte_ctx_create(int device, rte_ctx **out);
 /* also destroys the media still attached to the context (their handles become invalid) */
 int rte_ctx_destroy(rte_ctx *ctx);
-/* at most max_rhs (1..5, default 5; at most 3 above L = 5) beams on one medium share
- * a propagation launch on the mixed-precision cached path; the result bits do not
- * depend on it */
+/* at most max_rhs (1..5, default 5; at most 3 at L = 6 and 2 from L = 7) beams on one
+ * medium share a propagation launch on the mixed-precision cached path; the result bits
+ * do not depend on it */
 int rte_ctx_set_group_size(rte_ctx *ctx, int max_rhs);
 /* on != 0: time every launch from now on (resets the counters); 0: stop */
 int rte_ctx_profile(rte_ctx *ctx, int on);
