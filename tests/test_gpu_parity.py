@@ -152,13 +152,14 @@ def test_batch_equals_alone(ctx):
 DEFAULT_GROUP = 5  # rte_ctx_set_group_size's default
 
 
-@pytest.mark.parametrize("L", [3, 4, 5])
+@pytest.mark.parametrize("L", [3, 4, 5, 6, 7])
 def test_beams_sharing_a_medium_equal_alone(ctx, L):
     """Seven beams on one medium, propagated as groups of 5 and 2 (group size 5,
     the default), 4 and 3 (group size 4), 3, 3 and 1 (group size 3) or 2, 2, 2
     and 1 (group size 2) right-hand sides, give the same bits as each beam solved
     alone (L 3 and 5: the group of three reads W rows in halves; L 4: in whole
-    rows; groups of three to five prefetch their rows)."""
+    rows; groups of three to five prefetch their rows; L 6 and 7: the library
+    caps the groups at 3 and 2)."""
     shape = (12, 12, 12)
     bounds = bounds_for(shape)
     m = ctx.medium(shape, bounds, *hetero_maps(shape, seed=31), 0.9)
